@@ -1,0 +1,124 @@
+/*
+ * xm_amd.h - entry points of the MI355X build that have no counterpart in
+ * libxm's C API.  They sit beside the unchanged xm.h prototypes, so code
+ * written against libxm is unaffected by them.
+ */
+#ifndef XM_AMD_H_INCLUDED
+#define XM_AMD_H_INCLUDED
+
+#include <stdint.h>
+#include "tensor.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * Arithmetic used by xm_contract / xm_contract_backward for DOUBLE and
+ * DOUBLE_COMPLEX tensors.  No libxm counterpart: libxm always calls the
+ * BLAS routine of the element type (libxm src/contract.c:49-84).
+ *  NATIVE      - FP64 on the FP64 matrix cores (FP32 for FLOAT types).
+ *  SPLIT_BF16  - FP64 emulated by splitting each operand row/column into
+ *                BF16 slices that share one FP64 exponent, multiplying the
+ *                slices on BF16 matrix cores with FP32 accumulation over
+ *                K-chunks short enough to stay exact, and recombining in
+ *                FP64.  Exponents, scaling, recombination, alpha/beta and
+ *                the C update stay FP64.
+ */
+typedef enum {
+	XM_PRECISION_NATIVE = 0,
+	XM_PRECISION_SPLIT_BF16 = 1,
+} xm_precision_t;
+
+void xm_set_contract_precision(xm_precision_t precision);
+xm_precision_t xm_get_contract_precision(void);
+/* Number of BF16 slices per FP64 operand in SPLIT_BF16 mode (3..8). */
+void xm_set_split_slices(int nslices);
+int xm_get_split_slices(void);
+
+/*
+ * Figures of the most recent xm_contract / xm_contract_backward call.
+ * No libxm counterpart (libxm has no timers or counters).
+ * flops counts 2*m*n*k real flops (x4 for complex types) over the block
+ * pairs that survive libxm's pair merge with a nonzero factor, i.e. the
+ * GEMM work libxm itself executes (libxm src/contract.c:161-218).
+ */
+typedef struct {
+	double flops;		/* after the merge */
+	double flops_pre;	/* before the merge */
+	size_t npairs;		/* merged pairs with nonzero factor */
+	size_t npairs_pre;	/* pairs with two nonzero blocks, before merge */
+	size_t nblocks_c;	/* canonical C blocks written */
+	size_t nunfold;		/* operand panels built by the permute kernel */
+	double unfold_bytes;	/* bytes read + written by permute kernels */
+	double gemm_ms;		/* device time of the grouped GEMM launches */
+	double prep_ms;		/* device time of unfold / slice launches */
+	double post_ms;		/* device time of fold / scale launches */
+	double total_ms;	/* host wall time of the call */
+	int plan_cached;	/* 1 if the execution plan was reused */
+	int precision;		/* xm_precision_t used */
+	int nslices;		/* slices per operand (SPLIT_BF16 only) */
+	int gemm_launches;	/* number of grouped GEMM launches */
+} xm_contract_stats_t;
+
+void xm_contract_get_stats(xm_contract_stats_t *stats);
+
+/*
+ * The block pairs xm_contract multiplies for each canonical block of c, in
+ * libxm's order (canonical C blocks first-index-fastest, then contracted
+ * block combinations first-index-fastest), after libxm's pair merge and
+ * with zero-factor pairs dropped (libxm src/contract.c:161-218).
+ * m, n, k are the logical GEMM sizes: m over c's indices taken from a, n
+ * over c's indices taken from b, k over the contracted indices.
+ * Writes at most maxpairs entries; returns the total number of pairs.
+ * No libxm counterpart (the list is internal to libxm's compute_block).
+ */
+typedef struct {
+	xm_dim_t blkidxc, blkidxa, blkidxb;
+	xm_scalar_t alpha;	/* merged s_a * s_b, in the tensor's precision */
+	size_t m, n, k;
+} xm_contract_pair_t;
+
+size_t xm_contract_get_pairs(const xm_tensor_t *a, const xm_tensor_t *b,
+    const xm_tensor_t *c, const char *idxa, const char *idxb,
+    const char *idxc, xm_contract_pair_t *pairs, size_t maxpairs);
+
+/*
+ * Adjoint of xm_contract.  No libxm counterpart.
+ * For the forward c = alpha * a * b + beta * c and the loss
+ * L = sum over the CANONICAL entries of c of dc * c, writes
+ *   da = dL/d(canonical data of a),   db = dL/d(canonical data of b).
+ * Derivative blocks of dc count as zero.  Every block X of a that is used
+ * by the forward contributes through its source canonical block:
+ *   da[src(X)](permute(e, p_X)) += s_X * G_X(e),  G = alpha * dc (x) b,
+ * and likewise for b.  da / db must have the block structure of a / b
+ * (canonical blocks at the same places); either may be NULL.
+ */
+void xm_contract_backward(xm_scalar_t alpha, const xm_tensor_t *a,
+    const xm_tensor_t *b, const xm_tensor_t *dc, xm_tensor_t *da,
+    xm_tensor_t *db, const char *idxa, const char *idxb, const char *idxc);
+
+/*
+ * Fill every canonical block of t with reproducible pseudo-random values
+ * generated on the device: x = u * 10^w, u ~ U[lo, hi), w ~ U[wlo, whi)
+ * (wlo == whi == 0 gives plain uniform values).  Real and imaginary parts
+ * are drawn independently.  Values depend only on (seed, block index,
+ * element index).  No libxm counterpart (workload generation).
+ */
+void xm_amd_fill_random(xm_tensor_t *t, uint64_t seed, double lo, double hi,
+    double wlo, double whi);
+
+/* Bytes of device (HBM) and pinned-host memory held by an allocator's
+ * arena.  No libxm counterpart. */
+void xm_amd_allocator_usage(xm_allocator_t *allocator, size_t *device_bytes,
+    size_t *host_bytes);
+
+/* Block until all work issued by the library has completed; returns 0.
+ * No libxm counterpart (every xm_* operation is already synchronous). */
+int xm_amd_synchronize(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* XM_AMD_H_INCLUDED */
