@@ -1,0 +1,807 @@
+/*
+ * contract.cpp - xm_contract on the GPU (libxm src/contract.c:86-314).
+ *
+ * Host: the same checks and fatal messages as libxm, the same pair
+ * enumeration and merge for every canonical C block, then a plan:
+ *   - every surviving pair becomes a GEMM term  C(p,q) += al * X(p,k) Y(q,k)
+ *     of a grouped GEMM whose K dimension concatenates all pairs of a C
+ *     block (one owner per output tile, no atomics);
+ *   - operands whose block layout already is a P x K (or K x P) matrix are
+ *     read in place; the others are permuted once per contraction into a
+ *     panel by the permute kernel and shared by every pair that uses them;
+ *   - C blocks whose layout is not a matrix are permuted into a panel and
+ *     back.
+ * Plans are cached per (tensor structures, index strings, precision) and
+ * hold their descriptor lists in device memory; alpha and beta are read by
+ * the kernels from device memory, so a plan is reused across calls.
+ */
+#include "internal.h"
+#include "contract_plan.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <list>
+#include <map>
+#include <unordered_map>
+
+/* ---- precision selection ----------------------------------------- */
+
+static int g_precision = XM_PRECISION_NATIVE;
+static int g_nslices = 5;
+
+XM_API void
+xm_set_contract_precision(xm_precision_t p)
+{
+	if (p != XM_PRECISION_NATIVE && p != XM_PRECISION_SPLIT_BF16)
+		XM_FATAL("xm_set_contract_precision", "unknown precision");
+	g_precision = p;
+}
+
+XM_API xm_precision_t
+xm_get_contract_precision(void)
+{
+	return (xm_precision_t)g_precision;
+}
+
+XM_API void
+xm_set_split_slices(int n)
+{
+	if (n < 3 || n > 8)
+		XM_FATAL("xm_set_split_slices", "slices must be in 3..8");
+	g_nslices = n;
+}
+
+XM_API int
+xm_get_split_slices(void)
+{
+	return g_nslices;
+}
+
+int current_precision(void) { return g_precision; }
+int current_nslices(void) { return g_nslices; }
+
+/* ---- checks and masks (libxm src/contract.c:237-280) -------------- */
+
+void
+contract_masks(const char *fn, const xm_tensor_t *a, const xm_tensor_t *b,
+    const xm_tensor_t *c, const char *idxa, const char *idxb,
+    const char *idxc, masks_t &mk)
+{
+	if (a->alloc != c->alloc || b->alloc != c->alloc)
+		XM_FATAL(fn, "tensors must use same allocator");
+	if (a->type != c->type || b->type != c->type)
+		XM_FATAL(fn, "tensors must have same scalar type");
+	if (strlen(idxa) != a->bs->dims.n || strlen(idxb) != b->bs->dims.n ||
+	    strlen(idxc) != c->bs->dims.n)
+		XM_FATAL(fn, "bad contraction indices");
+	xm_make_masks(idxa, idxb, &mk.cidxa, &mk.cidxb);
+	xm_make_masks(idxc, idxa, &mk.cidxc, &mk.aidxa);
+	xm_make_masks(idxc, idxb, &mk.aidxc, &mk.aidxb);
+	if (mk.aidxa.n + mk.cidxa.n != a->bs->dims.n ||
+	    mk.aidxb.n + mk.cidxb.n != b->bs->dims.n ||
+	    mk.aidxc.n + mk.cidxc.n != c->bs->dims.n)
+		XM_FATAL(fn, "bad contraction indices");
+	if (!(mk.aidxc.n > 0 && mk.aidxc.i[0] == 0) &&
+	    !(mk.cidxc.n > 0 && mk.cidxc.i[0] == 0))
+		XM_FATAL(fn, "bad contraction indices");
+	for (size_t k = 0; k < mk.cidxa.n; k++)
+		if (!xm_block_space_eq1(a->bs, mk.cidxa.i[k], b->bs,
+		    mk.cidxb.i[k]))
+			XM_FATAL(fn, "inconsistent a and b tensor block-spaces");
+	for (size_t k = 0; k < mk.cidxc.n; k++)
+		if (!xm_block_space_eq1(c->bs, mk.cidxc.i[k], a->bs,
+		    mk.aidxa.i[k]))
+			XM_FATAL(fn, "inconsistent a and c tensor block-spaces");
+	for (size_t k = 0; k < mk.aidxc.n; k++)
+		if (!xm_block_space_eq1(c->bs, mk.aidxc.i[k], b->bs,
+		    mk.aidxb.i[k]))
+			XM_FATAL(fn, "inconsistent b and c tensor block-spaces");
+}
+
+/* ---- pair enumeration and merge (libxm src/contract.c:86-192) ----- */
+
+/* libxm src/contract.c:86-115: same source data for a and for b, and the
+ * same permutation at the free positions of a and of b. */
+static bool
+same_contraction(const xm_tensor_t *a, const xm_tensor_t *b,
+    const block_pair &x, const block_pair &y, const masks_t &mk)
+{
+	const xm_block &xa = a->blocks[x.fa], &ya = a->blocks[y.fa];
+	const xm_block &xb = b->blocks[x.fb], &yb = b->blocks[y.fb];
+	uint64_t pxa = xa.type == XM_BLOCK_TYPE_DERIVATIVE ?
+	    a->blocks[xa.data].data : xa.data;
+	uint64_t pya = ya.type == XM_BLOCK_TYPE_DERIVATIVE ?
+	    a->blocks[ya.data].data : ya.data;
+	uint64_t pxb = xb.type == XM_BLOCK_TYPE_DERIVATIVE ?
+	    b->blocks[xb.data].data : xb.data;
+	uint64_t pyb = yb.type == XM_BLOCK_TYPE_DERIVATIVE ?
+	    b->blocks[yb.data].data : yb.data;
+	if (pxa != pya || pxb != pyb)
+		return false;
+	for (size_t k = 0; k < mk.aidxa.n; k++)
+		if (xa.perm.i[mk.aidxa.i[k]] != ya.perm.i[mk.aidxa.i[k]])
+			return false;
+	for (size_t k = 0; k < mk.aidxb.n; k++)
+		if (xb.perm.i[mk.aidxb.i[k]] != yb.perm.i[mk.aidxb.i[k]])
+			return false;
+	return true;
+}
+
+/* libxm src/contract.c:161-192 for one C block.  `all` receives every
+ * contracted combination (alpha 0 for zero blocks and merged-away pairs),
+ * `members[i]` the combinations merged into survivor i.  Returns the
+ * number of combinations with two nonzero blocks. */
+size_t
+enumerate_pairs(const xm_tensor_t *a, const xm_tensor_t *b,
+    const xm_dim_t &blkc, const masks_t &mk, std::vector<block_pair> &all,
+    std::vector<std::vector<uint32_t>> *members)
+{
+	xm_dim_t nba = a->bs->nblocks, nbb = b->bs->nblocks;
+	size_t n = xm_dim_dot_mask(&nba, &mk.cidxa), npre = 0;
+	xm_dim_t ia = xm_dim_zero(nba.n), ib = xm_dim_zero(nbb.n);
+
+	xm_dim_set_mask(&ia, &mk.aidxa, &blkc, &mk.cidxc);
+	xm_dim_set_mask(&ib, &mk.aidxb, &blkc, &mk.aidxc);
+	all.resize(n);
+	for (size_t i = 0; i < n; i++) {
+		block_pair &p = all[i];
+		p.fa = xm_dim_offset(&ia, &nba);
+		p.fb = xm_dim_offset(&ib, &nbb);
+		p.alpha = 0;
+		const xm_block &ba = a->blocks[p.fa], &bb = b->blocks[p.fb];
+		if (ba.type != XM_BLOCK_TYPE_ZERO &&
+		    bb.type != XM_BLOCK_TYPE_ZERO) {
+			p.alpha = scalar_mul(ba.scalar, bb.scalar, a->type);
+			npre++;
+		}
+		p.nonzero = ba.type != XM_BLOCK_TYPE_ZERO &&
+		    bb.type != XM_BLOCK_TYPE_ZERO;
+		xm_dim_inc_mask(&ia, &nba, &mk.cidxa);
+		xm_dim_inc_mask(&ib, &nbb, &mk.cidxb);
+	}
+	if (members) {
+		members->assign(n, {});
+	}
+	for (size_t i = 0; i < n; i++) {
+		if (all[i].alpha == xm_scalar_t(0))
+			continue;
+		for (size_t j = i + 1; j < n; j++) {
+			if (all[j].alpha == xm_scalar_t(0))
+				continue;
+			if (same_contraction(a, b, all[i], all[j], mk)) {
+				all[i].alpha = scalar_add(all[i].alpha,
+				    all[j].alpha, a->type);
+				all[j].alpha = 0;
+				if (members)
+					(*members)[i].push_back((uint32_t)j);
+			}
+		}
+	}
+	return npre;
+}
+
+/* ---- operand layout analysis -------------------------------------- */
+
+/* Source stride of every logical index of a block (the permutation of a
+ * derivative block folded in): X(e) = S(permute(e, p)). */
+static void
+logical_strides(const xm_tensor_t *t, size_t flat, uint64_t *ls,
+    xm_dim_t *dims)
+{
+	const xm_block &b = t->blocks[flat];
+	xm_dim_t d = block_dims_flat(t, flat);
+	xm_dim_t sd = xm_dim_permute(&d, &b.perm);
+	uint64_t ss[XM_MAX_DIM], s = 1;
+	for (size_t k = 0; k < sd.n; k++) {
+		ss[k] = s;
+		s *= sd.i[k];
+	}
+	for (size_t k = 0; k < d.n; k++)
+		ls[k] = ss[b.perm.i[k]];
+	*dims = d;
+}
+
+/* Are the dims `order` (in order) laid out contiguously, the first one
+ * with stride `start`?  Extent-1 dims are ignored. */
+static bool
+contiguous_run(const uint64_t *ls, const xm_dim_t &d, const xm_dim_t &order,
+    uint64_t start, uint64_t *end)
+{
+	uint64_t want = start;
+	for (size_t k = 0; k < order.n; k++) {
+		size_t q = order.i[k];
+		if (d.i[q] == 1)
+			continue;
+		if (ls[q] != want)
+			return false;
+		want *= d.i[q];
+	}
+	*end = want;
+	return true;
+}
+
+/* Layout of a block as a (P x K) operand with free dims `F` (ordered as
+ * the matching C dims) and contracted dims `Kd`.  Returns true when the
+ * block can be read in place and sets mode/ld. */
+static bool
+direct_layout(const uint64_t *ls, const xm_dim_t &d, const xm_dim_t &F,
+    const xm_dim_t &Kd, uint8_t *mode, uint32_t *ld)
+{
+	uint64_t P = xm_dim_dot_mask(&d, &F), K = xm_dim_dot_mask(&d, &Kd), e;
+	if (contiguous_run(ls, d, F, 1, &e) && contiguous_run(ls, d, Kd, P, &e)) {
+		*mode = PANEL_PCONTIG;
+		*ld = (uint32_t)std::max<uint64_t>(P, 1);
+		return true;
+	}
+	if (contiguous_run(ls, d, Kd, 1, &e) && contiguous_run(ls, d, F, K, &e)) {
+		*mode = PANEL_KCONTIG;
+		*ld = (uint32_t)std::max<uint64_t>(K, 1);
+		return true;
+	}
+	return false;
+}
+
+/* sigma test for substituting a merged member r for survivor i: the
+ * contracted index t' of r maps to the source dim perm_r[cidx[t']], which
+ * must be the source dim of the same contracted index of i for both
+ * operands, so that both sums run over the same products. */
+static bool
+same_k_order(const xm_block &xi, const xm_block &xr, const xm_dim_t &cidx,
+    int *sigma)
+{
+	for (size_t t = 0; t < cidx.n; t++) {
+		int found = -1;
+		for (size_t u = 0; u < cidx.n; u++)
+			if (xr.perm.i[cidx.i[t]] == xi.perm.i[cidx.i[u]])
+				found = (int)u;
+		if (found < 0)
+			return false;
+		sigma[t] = found;
+	}
+	return true;
+}
+
+static bool
+substitutable(const xm_tensor_t *a, const xm_tensor_t *b, const masks_t &mk,
+    const block_pair &i, const block_pair &r)
+{
+	int sa[XM_MAX_DIM], sb[XM_MAX_DIM];
+	if (!same_k_order(a->blocks[i.fa], a->blocks[r.fa], mk.cidxa, sa) ||
+	    !same_k_order(b->blocks[i.fb], b->blocks[r.fb], mk.cidxb, sb))
+		return false;
+	for (size_t t = 0; t < mk.cidxa.n; t++)
+		if (sa[t] != sb[t])
+			return false;
+	return true;
+}
+
+/* ---- plan building ------------------------------------------------- */
+
+struct panel_key {
+	uint64_t src;
+	uint32_t n;
+	uint64_t ext[XM_MAX_DIM], str[XM_MAX_DIM];
+	bool operator<(const panel_key &o) const {
+		if (src != o.src) return src < o.src;
+		if (n != o.n) return n < o.n;
+		for (uint32_t k = 0; k < n; k++) {
+			if (ext[k] != o.ext[k]) return ext[k] < o.ext[k];
+			if (str[k] != o.str[k]) return str[k] < o.str[k];
+		}
+		return false;
+	}
+};
+
+struct plan_builder {
+	contract_plan *pl;
+	size_t esz;
+	std::map<panel_key, uint64_t> panels;	/* key -> workspace offset */
+
+	/* Operand of one pair: in place, or a (shared) panel. */
+	void operand(const xm_tensor_t *t, size_t flat, const xm_dim_t &F,
+	    const xm_dim_t &Kd, uint64_t *ptr, uint8_t *mode, uint32_t *ld,
+	    bool *in_ws)
+	{
+		uint64_t ls[XM_MAX_DIM];
+		xm_dim_t d;
+		logical_strides(t, flat, ls, &d);
+		const xm_block &b = t->blocks[flat];
+		uint64_t src = b.type == XM_BLOCK_TYPE_DERIVATIVE ?
+		    t->blocks[b.data].data : b.data;
+		*in_ws = false;
+		if (direct_layout(ls, d, F, Kd, mode, ld)) {
+			*ptr = src;
+			return;
+		}
+		/* Permuted panel; P-contiguous unless the source's
+		 * stride-1 index is a contracted one. */
+		bool kfirst = false;
+		for (size_t k = 0; k < Kd.n; k++)
+			if (ls[Kd.i[k]] == 1 && d.i[Kd.i[k]] > 1)
+				kfirst = true;
+		panel_key key;
+		key.src = src;
+		key.n = 0;
+		const xm_dim_t &o1 = kfirst ? Kd : F, &o2 = kfirst ? F : Kd;
+		for (size_t k = 0; k < o1.n; k++) {
+			key.ext[key.n] = d.i[o1.i[k]];
+			key.str[key.n++] = ls[o1.i[k]];
+		}
+		for (size_t k = 0; k < o2.n; k++) {
+			key.ext[key.n] = d.i[o2.i[k]];
+			key.str[key.n++] = ls[o2.i[k]];
+		}
+		uint64_t P = xm_dim_dot_mask(&d, &F), K = xm_dim_dot_mask(&d, &Kd);
+		*mode = kfirst ? PANEL_KCONTIG : PANEL_PCONTIG;
+		*ld = (uint32_t)std::max<uint64_t>(kfirst ? K : P, 1);
+		auto it = panels.find(key);
+		if (it != panels.end()) {
+			*ptr = it->second;
+			*in_ws = true;
+			return;
+		}
+		uint64_t off = pl->ws_bytes;
+		pl->ws_bytes += (P * K * esz + 255) / 256 * 256;
+		panels[key] = off;
+		elem_job j;
+		memset(&j, 0, sizeof j);
+		j.dst = off;		/* relocated once the workspace exists */
+		j.src = src;
+		j.n = P * K;
+		j.ndim = key.n;
+		for (uint32_t k = 0; k < key.n; k++) {
+			j.dims[k] = (uint32_t)key.ext[k];
+			j.sstride[k] = key.str[k];
+		}
+		pl->unfold_jobs.push_back(j);
+		pl->unfold_bytes += 2.0 * P * K * esz;
+		*ptr = off;
+		*in_ws = true;
+	}
+};
+
+static void
+finish_chunks(std::vector<elem_job> &jobs, size_t *nchunks)
+{
+	size_t c = 0;
+	for (auto &j : jobs) {
+		j.first_chunk = c;
+		c += (j.n + 4095) / 4096;
+	}
+	*nchunks = c;
+}
+
+static contract_plan *
+build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
+    const masks_t &mk, int precision)
+{
+	contract_plan *pl = new contract_plan;
+	plan_builder pb;
+	pb.pl = pl;
+	pb.esz = elem_size(c->type);
+	pl->type = c->type;
+	pl->precision = precision;
+	size_t mult = is_complex(c->type) ? 4 : 1;
+
+	xm_dim_t *list;
+	size_t nlist;
+	xm_tensor_get_canonical_block_list(c, &list, &nlist);
+	pl->nblocks_c = nlist;
+
+	std::vector<block_pair> all;
+	std::vector<std::vector<uint32_t>> members;
+	for (size_t ic = 0; ic < nlist; ic++) {
+		const xm_dim_t blkc = list[ic];
+		size_t fc = block_flat(c, blkc);
+		xm_dim_t dc = block_dims_flat(c, fc);
+		uint64_t m = xm_dim_dot_mask(&dc, &mk.cidxc);
+		uint64_t n = xm_dim_dot_mask(&dc, &mk.aidxc);
+
+		/* orientation of the C block */
+		uint64_t lsc[XM_MAX_DIM];
+		xm_dim_t dd;
+		logical_strides(c, fc, lsc, &dd);
+		uint64_t e;
+		bool swap;	/* P = n (X = B), Q = m (Y = A) */
+		bool cpanel = false;
+		if (contiguous_run(lsc, dc, mk.cidxc, 1, &e) &&
+		    contiguous_run(lsc, dc, mk.aidxc, std::max<uint64_t>(m, 1), &e))
+			swap = false;
+		else if (contiguous_run(lsc, dc, mk.aidxc, 1, &e) &&
+		    contiguous_run(lsc, dc, mk.cidxc, std::max<uint64_t>(n, 1), &e))
+			swap = true;
+		else {
+			swap = false;
+			cpanel = true;
+		}
+
+		gemm_cblock cb;
+		memset(&cb, 0, sizeof cb);
+		cb.p = (uint32_t)(swap ? n : m);
+		cb.q = (uint32_t)(swap ? m : n);
+		cb.ldc = std::max<uint32_t>(cb.p, 1);
+		cb.pair_begin = (uint32_t)pl->pairs.size();
+		uint64_t cptr = c->blocks[fc].data;
+		if (cpanel) {
+			/* C panel: rows = cidxc dims, cols = aidxc dims */
+			uint64_t off = pl->ws_bytes;
+			pl->ws_bytes += (m * n * pb.esz + 255) / 256 * 256;
+			elem_job in, out;
+			memset(&in, 0, sizeof in);
+			in.dst = off;
+			in.src = cptr;
+			in.n = m * n;
+			for (size_t k = 0; k < mk.cidxc.n; k++) {
+				in.dims[in.ndim] = (uint32_t)dc.i[mk.cidxc.i[k]];
+				in.sstride[in.ndim++] = lsc[mk.cidxc.i[k]];
+			}
+			for (size_t k = 0; k < mk.aidxc.n; k++) {
+				in.dims[in.ndim] = (uint32_t)dc.i[mk.aidxc.i[k]];
+				in.sstride[in.ndim++] = lsc[mk.aidxc.i[k]];
+			}
+			/* fold: C block element (canonical order) <- panel */
+			memset(&out, 0, sizeof out);
+			out.dst = cptr;
+			out.src = off;
+			out.n = m * n;
+			out.ndim = (uint32_t)dc.n;
+			uint64_t pstride[XM_MAX_DIM], s = 1;
+			for (size_t k = 0; k < mk.cidxc.n; k++) {
+				pstride[mk.cidxc.i[k]] = s;
+				s *= dc.i[mk.cidxc.i[k]];
+			}
+			for (size_t k = 0; k < mk.aidxc.n; k++) {
+				pstride[mk.aidxc.i[k]] = s;
+				s *= dc.i[mk.aidxc.i[k]];
+			}
+			for (size_t k = 0; k < dc.n; k++) {
+				out.dims[k] = (uint32_t)dc.i[k];
+				out.sstride[k] = pstride[k];
+			}
+			pl->cin_jobs.push_back(in);
+			pl->cout_jobs.push_back(out);
+			pl->unfold_bytes += 4.0 * m * n * pb.esz;
+			cb.c = off;
+			cb.flags |= 1;
+		} else {
+			cb.c = cptr;
+		}
+
+		size_t npre = enumerate_pairs(a, b, blkc, mk, all, &members);
+		pl->npairs_pre += npre;
+		for (size_t i = 0; i < all.size(); i++) {
+			if (!all[i].nonzero)
+				continue;
+			xm_dim_t da = block_dims_flat(a, all[i].fa);
+			uint64_t k = xm_dim_dot_mask(&da, &mk.cidxa);
+			pl->flops_pre += 2.0 * mult * m * n * k;
+			if (all[i].alpha == xm_scalar_t(0))
+				continue;
+			pl->flops += 2.0 * mult * m * n * k;
+			pl->npairs++;
+			pl->pair_list.push_back({fc, all[i].fa, all[i].fb,
+			    all[i].alpha, m, n, k});
+
+			/* pick the cheapest equivalent member of the merge
+			 * group (fewest operands needing a permute) */
+			block_pair best = all[i];
+			int best_cost = 3;
+			std::vector<uint32_t> cand = {(uint32_t)i};
+			for (uint32_t r : members[i])
+				cand.push_back(r);
+			for (uint32_t r : cand) {
+				if (r != i && !substitutable(a, b, mk, all[i], all[r]))
+					continue;
+				uint64_t ls[XM_MAX_DIM];
+				xm_dim_t d;
+				uint8_t md;
+				uint32_t ld;
+				int cost = 0;
+				logical_strides(a, all[r].fa, ls, &d);
+				cost += !direct_layout(ls, d, mk.aidxa, mk.cidxa, &md, &ld);
+				logical_strides(b, all[r].fb, ls, &d);
+				cost += !direct_layout(ls, d, mk.aidxb, mk.cidxb, &md, &ld);
+				if (cost < best_cost) {
+					best_cost = cost;
+					best = all[r];
+				}
+			}
+			if (best.fa != all[i].fa || best.fb != all[i].fb)
+				pl->nsubstituted++;
+
+			gemm_pair gp;
+			memset(&gp, 0, sizeof gp);
+			uint64_t pa, pbp;
+			uint8_t ma, mb;
+			uint32_t lda, ldb;
+			bool wa, wb;
+			pb.operand(a, best.fa, mk.aidxa, mk.cidxa, &pa, &ma, &lda, &wa);
+			pb.operand(b, best.fb, mk.aidxb, mk.cidxb, &pbp, &mb, &ldb, &wb);
+			gp.k = (uint32_t)k;
+			gp.a_re = all[i].alpha.real();
+			gp.a_im = all[i].alpha.imag();
+			if (!swap) {
+				gp.x = pa; gp.xmode = ma; gp.ldx = lda;
+				gp.y = pbp; gp.ymode = mb; gp.ldy = ldb;
+				pl->x_in_ws.push_back(wa);
+				pl->y_in_ws.push_back(wb);
+			} else {
+				gp.x = pbp; gp.xmode = mb; gp.ldx = ldb;
+				gp.y = pa; gp.ymode = ma; gp.ldy = lda;
+				pl->x_in_ws.push_back(wb);
+				pl->y_in_ws.push_back(wa);
+			}
+			pl->pairs.push_back(gp);
+			/* split path: which operand is A (kept for the
+			 * slicing stage) */
+			pl->pair_swap.push_back(swap);
+		}
+		cb.pair_end = (uint32_t)pl->pairs.size();
+		pl->cblocks.push_back(cb);
+		pl->cpanel.push_back(cpanel);
+	}
+	free(list);
+	pl->nunfold = pl->unfold_jobs.size();
+	finish_chunks(pl->unfold_jobs, &pl->unfold_chunks);
+	finish_chunks(pl->cin_jobs, &pl->cin_chunks);
+	finish_chunks(pl->cout_jobs, &pl->cout_chunks);
+	return pl;
+}
+
+/* ---- plan cache ---------------------------------------------------- */
+
+struct plan_key {
+	uint64_t ida, va, idb, vb, idc, vc;
+	std::string ia, ib, ic;
+	int precision, nslices;
+	bool operator<(const plan_key &o) const {
+		return std::tie(ida, va, idb, vb, idc, vc, ia, ib, ic, precision,
+		    nslices) < std::tie(o.ida, o.va, o.idb, o.vb, o.idc, o.vc,
+		    o.ia, o.ib, o.ic, o.precision, o.nslices);
+	}
+};
+
+static std::mutex cache_mu;
+static std::list<std::pair<plan_key, contract_plan *>> cache;
+static const size_t CACHE_MAX = 8;
+
+void
+plan_release(contract_plan *pl)
+{
+	if (pl == nullptr)
+		return;
+	HIP_OK("xm_contract", hipStreamSynchronize(xm_stream()));
+	if (pl->dev)
+		HIP_OK("xm_contract", hipFree(pl->dev));
+	if (pl->ws)
+		HIP_OK("xm_contract", hipFree(pl->ws));
+	if (pl->host_scalars)
+		HIP_OK("xm_contract", hipHostFree(pl->host_scalars));
+	if (pl->ev[0])
+		for (auto &e : pl->ev)
+			HIP_OK("xm_contract", hipEventDestroy(e));
+	split_release(pl);
+	delete pl;
+}
+
+/* Relocate workspace offsets to addresses and move every descriptor list
+ * to one device allocation. */
+static void
+plan_upload(contract_plan *pl)
+{
+	const char *fn = "xm_contract";
+	if (pl->ws_bytes)
+		HIP_OK(fn, hipMalloc(&pl->ws, pl->ws_bytes));
+	uint64_t wsb = (uint64_t)(uintptr_t)pl->ws;
+	for (auto &j : pl->unfold_jobs)
+		j.dst += wsb;
+	for (auto &j : pl->cin_jobs)
+		j.dst += wsb;
+	for (auto &j : pl->cout_jobs)
+		j.src += wsb;
+	for (size_t i = 0; i < pl->cblocks.size(); i++)
+		if (pl->cpanel[i])
+			pl->cblocks[i].c += wsb;
+	for (size_t i = 0; i < pl->pairs.size(); i++) {
+		if (pl->x_in_ws[i])
+			pl->pairs[i].x += wsb;
+		if (pl->y_in_ws[i])
+			pl->pairs[i].y += wsb;
+	}
+	build_tiles(pl);
+
+	/* one device allocation: scalars | cblocks | pairs | tiles | jobs */
+	std::vector<std::pair<const void *, size_t>> parts;
+	auto add = [&](const void *p, size_t n) { parts.push_back({p, n}); };
+	add(nullptr, sizeof(gemm_scalars));
+	add(pl->cblocks.data(), pl->cblocks.size() * sizeof(gemm_cblock));
+	add(pl->pairs.data(), pl->pairs.size() * sizeof(gemm_pair));
+	add(pl->tiles.data(), pl->tiles.size() * sizeof(gemm_tile));
+	std::vector<char> ju = elem_jobs_pack(pl->unfold_jobs);
+	std::vector<char> ji = elem_jobs_pack(pl->cin_jobs);
+	std::vector<char> jo = elem_jobs_pack(pl->cout_jobs);
+	add(ju.data(), ju.size());
+	add(ji.data(), ji.size());
+	add(jo.data(), jo.size());
+	size_t total = 0;
+	std::vector<size_t> offs;
+	for (auto &p : parts) {
+		offs.push_back(total);
+		total += (p.second + 255) / 256 * 256;
+	}
+	HIP_OK(fn, hipMalloc(&pl->dev, std::max<size_t>(total, 256)));
+	std::vector<char> host(total, 0);
+	for (size_t i = 0; i < parts.size(); i++)
+		if (parts[i].first)
+			memcpy(host.data() + offs[i], parts[i].first,
+			    parts[i].second);
+	HIP_OK(fn, hipMemcpy(pl->dev, host.data(), total,
+	    hipMemcpyHostToDevice));
+	char *base = (char *)pl->dev;
+	pl->d_scalars = (gemm_scalars *)(base + offs[0]);
+	pl->d_cblocks = (gemm_cblock *)(base + offs[1]);
+	pl->d_pairs = (gemm_pair *)(base + offs[2]);
+	pl->d_tiles = (gemm_tile *)(base + offs[3]);
+	pl->d_unfold = base + offs[4];
+	pl->d_cin = base + offs[5];
+	pl->d_cout = base + offs[6];
+	HIP_OK(fn, hipHostMalloc(&pl->host_scalars, sizeof(gemm_scalars),
+	    hipHostMallocDefault));
+	for (auto &e : pl->ev)
+		HIP_OK(fn, hipEventCreate(&e));
+}
+
+static contract_plan *
+get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
+    const char *ia, const char *ib, const char *ic, const masks_t &mk,
+    bool *cached)
+{
+	int prec = current_precision();
+	if (c->type != XM_SCALAR_DOUBLE)
+		prec = XM_PRECISION_NATIVE;
+	plan_key key{a->id, a->version, b->id, b->version, c->id, c->version,
+	    ia, ib, ic, prec, prec == XM_PRECISION_SPLIT_BF16 ?
+	    current_nslices() : 0};
+	std::lock_guard<std::mutex> g(cache_mu);
+	for (auto it = cache.begin(); it != cache.end(); ++it)
+		if (!(it->first < key) && !(key < it->first)) {
+			cache.splice(cache.begin(), cache, it);
+			*cached = true;
+			return cache.front().second;
+		}
+	*cached = false;
+	contract_plan *pl = build_plan(a, b, c, mk, prec);
+	pl->nslices = key.nslices;
+	if (prec == XM_PRECISION_SPLIT_BF16)
+		split_prepare(pl, a, b, c, mk);
+	plan_upload(pl);
+	cache.push_front({key, pl});
+	while (cache.size() > CACHE_MAX) {
+		plan_release(cache.back().second);
+		cache.pop_back();
+	}
+	return pl;
+}
+
+/* Drop every cached plan that references tensor id `id` (called when a
+ * tensor is freed so its storage can be reused safely). */
+void
+plan_cache_clear(void)
+{
+	std::lock_guard<std::mutex> g(cache_mu);
+	for (auto &p : cache)
+		plan_release(p.second);
+	cache.clear();
+}
+
+/* ---- execution ------------------------------------------------------ */
+
+static xm_contract_stats_t last_stats;
+
+static void
+run_plan(contract_plan *pl, xm_scalar_t alpha, xm_scalar_t beta)
+{
+	const char *fn = "xm_contract";
+	hipStream_t s = xm_stream();
+	gemm_scalars sc;
+	sc.alpha_re = alpha.real();
+	sc.alpha_im = alpha.imag();
+	sc.beta_re = beta.real();
+	sc.beta_im = beta.imag();
+	/* alpha == 0: C = beta*C only (libxm src/contract.c:159-160) */
+	sc.flags = alpha == xm_scalar_t(0) ? 1 : 0;
+	sc.flags |= beta == xm_scalar_t(0) ? 2 : 0;
+	*pl->host_scalars = sc;
+	HIP_OK(fn, hipMemcpyAsync(pl->d_scalars, pl->host_scalars, sizeof sc,
+	    hipMemcpyHostToDevice, s));
+	bool do_pairs = !(sc.flags & 1);
+	HIP_OK(fn, hipEventRecord(pl->ev[0], s));
+	if (do_pairs && pl->unfold_chunks)
+		launch_elem_packed(pl->d_unfold, pl->unfold_jobs.size(),
+		    pl->unfold_chunks, ELEM_GATHER, pl->type, pl->type, 0, s);
+	if (!(sc.flags & 2) && pl->cin_chunks)
+		launch_elem_packed(pl->d_cin, pl->cin_jobs.size(),
+		    pl->cin_chunks, ELEM_GATHER, pl->type, pl->type, 0, s);
+	if (do_pairs && pl->precision == XM_PRECISION_SPLIT_BF16)
+		split_slices(pl, s);
+	HIP_OK(fn, hipEventRecord(pl->ev[1], s));
+	if (pl->precision == XM_PRECISION_SPLIT_BF16)
+		split_gemm(pl, s);
+	else
+		launch_gemm(pl, s);
+	HIP_OK(fn, hipEventRecord(pl->ev[2], s));
+	if (pl->cout_chunks)
+		launch_elem_packed(pl->d_cout, pl->cout_jobs.size(),
+		    pl->cout_chunks, ELEM_GATHER, pl->type, pl->type, 0, s);
+	HIP_OK(fn, hipEventRecord(pl->ev[3], s));
+	HIP_OK(fn, hipStreamSynchronize(s));
+}
+
+XM_API void
+xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
+    xm_scalar_t beta, xm_tensor_t *c, const char *idxa, const char *idxb,
+    const char *idxc)
+{
+	auto t0 = std::chrono::steady_clock::now();
+	masks_t mk;
+	contract_masks("xm_contract", a, b, c, idxa, idxb, idxc, mk);
+	bool cached = false;
+	contract_plan *pl = get_plan(a, b, c, idxa, idxb, idxc, mk, &cached);
+	run_plan(pl, alpha, beta);
+	auto t1 = std::chrono::steady_clock::now();
+	xm_contract_stats_t &st = last_stats;
+	memset(&st, 0, sizeof st);
+	st.flops = pl->flops;
+	st.flops_pre = pl->flops_pre;
+	st.npairs = pl->npairs;
+	st.npairs_pre = pl->npairs_pre;
+	st.nblocks_c = pl->nblocks_c;
+	st.nunfold = pl->nunfold;
+	st.unfold_bytes = pl->unfold_bytes;
+	st.prep_ms = elapsed_ms(pl->ev[0], pl->ev[1]);
+	st.gemm_ms = elapsed_ms(pl->ev[1], pl->ev[2]);
+	st.post_ms = elapsed_ms(pl->ev[2], pl->ev[3]);
+	st.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+	st.plan_cached = cached;
+	st.precision = pl->precision;
+	st.nslices = pl->nslices;
+	st.gemm_launches = pl->gemm_launches;
+}
+
+XM_API void
+xm_contract_get_stats(xm_contract_stats_t *s)
+{
+	*s = last_stats;
+}
+
+XM_API size_t
+xm_contract_get_pairs(const xm_tensor_t *a, const xm_tensor_t *b,
+    const xm_tensor_t *c, const char *idxa, const char *idxb,
+    const char *idxc, xm_contract_pair_t *out, size_t maxout)
+{
+	masks_t mk;
+	contract_masks("xm_contract_get_pairs", a, b, c, idxa, idxb, idxc, mk);
+	contract_plan *pl = build_plan(a, b, c, mk, XM_PRECISION_NATIVE);
+	size_t n = pl->pair_list.size();
+	for (size_t i = 0; i < n && i < maxout; i++) {
+		const plan_pair &p = pl->pair_list[i];
+		out[i].blkidxc = grid_index(c, p.fc);
+		out[i].blkidxa = grid_index(a, p.fa);
+		out[i].blkidxb = grid_index(b, p.fb);
+		out[i].alpha = p.alpha;
+		out[i].m = p.m;
+		out[i].n = p.n;
+		out[i].k = p.k;
+	}
+	memset(&last_stats, 0, sizeof last_stats);
+	last_stats.flops = pl->flops;
+	last_stats.flops_pre = pl->flops_pre;
+	last_stats.npairs = pl->npairs;
+	last_stats.npairs_pre = pl->npairs_pre;
+	last_stats.nblocks_c = pl->nblocks_c;
+	last_stats.nunfold = pl->nunfold;
+	delete pl;
+	return n;
+}

@@ -1,0 +1,92 @@
+/*
+ * contract_plan.h - the execution plan of one contraction (see
+ * contract.cpp) and the GEMM / split-precision stages that run it.
+ */
+#ifndef XM_CONTRACT_PLAN_H
+#define XM_CONTRACT_PLAN_H
+
+#include "internal.h"
+
+#include <vector>
+
+struct masks_t {
+	xm_dim_t cidxa, aidxa, cidxb, aidxb, cidxc, aidxc;
+};
+
+/* One contracted block combination of one C block. */
+struct block_pair {
+	size_t fa, fb;		/* flat block indices in a and b */
+	xm_scalar_t alpha;	/* s_a * s_b, merged; 0 if dropped */
+	bool nonzero;		/* both blocks nonzero before the merge */
+};
+
+/* A surviving pair as libxm reports it (libxm order). */
+struct plan_pair {
+	size_t fc, fa, fb;
+	xm_scalar_t alpha;
+	uint64_t m, n, k;
+};
+
+/* Split-precision stage state (split.hip). */
+struct split_state;
+
+struct contract_plan {
+	xm_scalar_type_t type;
+	int precision = XM_PRECISION_NATIVE;
+	int nslices = 0;
+
+	/* work */
+	double flops = 0, flops_pre = 0, unfold_bytes = 0;
+	size_t npairs = 0, npairs_pre = 0, nblocks_c = 0, nunfold = 0;
+	size_t nsubstituted = 0;
+	std::vector<plan_pair> pair_list;
+
+	/* grouped GEMM descriptors (host copies) */
+	std::vector<gemm_cblock> cblocks;
+	std::vector<bool> cpanel;
+	std::vector<gemm_pair> pairs;
+	std::vector<bool> x_in_ws, y_in_ws, pair_swap;
+	std::vector<gemm_tile> tiles;
+	int tile_cfg = 0;
+	int gemm_launches = 0;
+
+	/* permute jobs: operand panels, C in, C out */
+	std::vector<elem_job> unfold_jobs, cin_jobs, cout_jobs;
+	size_t unfold_chunks = 0, cin_chunks = 0, cout_chunks = 0;
+
+	/* device memory */
+	size_t ws_bytes = 0;
+	void *ws = nullptr;		/* panels */
+	void *dev = nullptr;		/* descriptors */
+	gemm_scalars *d_scalars = nullptr, *host_scalars = nullptr;
+	gemm_cblock *d_cblocks = nullptr;
+	gemm_pair *d_pairs = nullptr;
+	gemm_tile *d_tiles = nullptr;
+	void *d_unfold = nullptr, *d_cin = nullptr, *d_cout = nullptr;
+	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+	split_state *split = nullptr;
+};
+
+void contract_masks(const char *fn, const xm_tensor_t *a,
+    const xm_tensor_t *b, const xm_tensor_t *c, const char *idxa,
+    const char *idxb, const char *idxc, masks_t &mk);
+size_t enumerate_pairs(const xm_tensor_t *a, const xm_tensor_t *b,
+    const xm_dim_t &blkc, const masks_t &mk, std::vector<block_pair> &all,
+    std::vector<std::vector<uint32_t>> *members);
+int current_precision(void);
+int current_nslices(void);
+void plan_cache_clear(void);
+
+/* gemm.hip */
+void build_tiles(contract_plan *pl);
+void launch_gemm(contract_plan *pl, hipStream_t s);
+
+/* split.hip */
+void split_prepare(contract_plan *pl, const xm_tensor_t *a,
+    const xm_tensor_t *b, const xm_tensor_t *c, const masks_t &mk);
+void split_slices(contract_plan *pl, hipStream_t s);
+void split_gemm(contract_plan *pl, hipStream_t s);
+void split_release(contract_plan *pl);
+
+#endif /* XM_CONTRACT_PLAN_H */

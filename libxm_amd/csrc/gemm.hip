@@ -1,0 +1,366 @@
+/*
+ * gemm.hip - grouped GEMM over the block pairs of a contraction
+ * (replaces the per-pair ?gemm_ loop of libxm src/contract.c:193-216).
+ *
+ * One launch covers every canonical C block.  A work tile is
+ * (C block, P-range, Q-range); its K loop walks all surviving pairs of
+ * that C block back to back (K concatenation), so each output element has
+ * one owner and no atomics are needed.  Per pair the X tile is scaled by
+ * al = alpha * (s_a * s_b) while it is staged into LDS.  The epilogue
+ * applies beta: C = acc (beta == 0, C not read) or beta*C + acc.
+ *
+ * FP64: v_mfma_f64_16x16x4_f64.  The MFMA's A operand is the Y tile and
+ * its B operand the X tile, so D is laid out with P along the lanes and
+ * the stores to the P-contiguous C panel are coalesced.
+ * FP32 / complex: a register-tiled FMA kernel of the same structure.
+ */
+#include "contract_plan.h"
+
+#include <algorithm>
+#include <cstdlib>
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+#define GEMM_WG 256
+
+__device__ __forceinline__ double
+ld_x(const double *x, uint8_t mode, uint32_t ld, uint32_t p, uint32_t k,
+    uint32_t P, uint32_t K)
+{
+	if (p >= P || k >= K)
+		return 0.0;
+	return mode == PANEL_PCONTIG ? x[(size_t)p + (size_t)k * ld] :
+	    x[(size_t)k + (size_t)p * ld];
+}
+
+/*
+ * FP64 tile kernel.  BP x BQ outputs per workgroup, 4 waves as 2 x 2, each
+ * wave (BP/2) x (BQ/2) = MT x NT MFMA tiles of 16 x 16.  BK = 16.
+ */
+template <int BP, int BQ>
+__global__ void __launch_bounds__(GEMM_WG)
+k_gemm_f64(const gemm_tile *__restrict__ tiles,
+    const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
+    const gemm_scalars *__restrict__ scp)
+{
+	constexpr int BK = 16;
+	constexpr int MT = BP / 2 / 16, NT = BQ / 2 / 16;
+	constexpr int EX = BP * BK / GEMM_WG, EY = BQ * BK / GEMM_WG;
+	constexpr int LP = BP + 1, LQ = BQ + 1;
+	__shared__ double Xs[BK * LP];
+	__shared__ double Ys[BK * LQ];
+
+	const gemm_tile tl = tiles[blockIdx.x];
+	const gemm_cblock cb = cbs[tl.cblock];
+	const gemm_scalars sc = *scp;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int wp = (wave & 1) * (BP / 2), wq = (wave >> 1) * (BQ / 2);
+
+	d4 acc[MT][NT];
+	for (int i = 0; i < MT; i++)
+		for (int j = 0; j < NT; j++)
+			acc[i][j] = d4{0, 0, 0, 0};
+
+	const uint32_t pbeg = (sc.flags & 1) ? cb.pair_end : cb.pair_begin;
+	for (uint32_t ip = pbeg; ip < cb.pair_end; ip++) {
+		const gemm_pair pr = pairs[ip];
+		const double al = __dmul_rn(sc.alpha_re, pr.a_re);
+		const double *X = (const double *)(uintptr_t)pr.x;
+		const double *Y = (const double *)(uintptr_t)pr.y;
+		for (uint32_t k0 = 0; k0 < pr.k; k0 += BK) {
+			double rx[EX], ry[EY];
+#pragma unroll
+			for (int e = 0; e < EX; e++) {
+				int idx = tid + GEMM_WG * e, p, k;
+				if (pr.xmode == PANEL_PCONTIG) {
+					p = idx % BP; k = idx / BP;
+				} else {
+					k = idx % BK; p = idx / BK;
+				}
+				rx[e] = ld_x(X, pr.xmode, pr.ldx, tl.p0 + p, k0 + k,
+				    cb.p, pr.k) * al;
+			}
+#pragma unroll
+			for (int e = 0; e < EY; e++) {
+				int idx = tid + GEMM_WG * e, q, k;
+				if (pr.ymode == PANEL_PCONTIG) {
+					q = idx % BQ; k = idx / BQ;
+				} else {
+					k = idx % BK; q = idx / BK;
+				}
+				ry[e] = ld_x(Y, pr.ymode, pr.ldy, tl.q0 + q, k0 + k,
+				    cb.q, pr.k);
+			}
+			__syncthreads();
+#pragma unroll
+			for (int e = 0; e < EX; e++) {
+				int idx = tid + GEMM_WG * e, p, k;
+				if (pr.xmode == PANEL_PCONTIG) {
+					p = idx % BP; k = idx / BP;
+				} else {
+					k = idx % BK; p = idx / BK;
+				}
+				Xs[k * LP + p] = rx[e];
+			}
+#pragma unroll
+			for (int e = 0; e < EY; e++) {
+				int idx = tid + GEMM_WG * e, q, k;
+				if (pr.ymode == PANEL_PCONTIG) {
+					q = idx % BQ; k = idx / BQ;
+				} else {
+					k = idx % BK; q = idx / BK;
+				}
+				Ys[k * LQ + q] = ry[e];
+			}
+			__syncthreads();
+#pragma unroll
+			for (int kk = 0; kk < BK; kk += 4) {
+				const int kr = kk + (lane >> 4), c16 = lane & 15;
+				double xa[MT], yb[NT];
+#pragma unroll
+				for (int i = 0; i < MT; i++)
+					xa[i] = Xs[kr * LP + wp + i * 16 + c16];
+#pragma unroll
+				for (int j = 0; j < NT; j++)
+					yb[j] = Ys[kr * LQ + wq + j * 16 + c16];
+#pragma unroll
+				for (int i = 0; i < MT; i++)
+#pragma unroll
+					for (int j = 0; j < NT; j++)
+						acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+						    yb[j], xa[i], acc[i][j], 0, 0, 0);
+			}
+		}
+	}
+
+	/* D[q][p] of v_mfma_f64_16x16x4_f64: lane -> p = lane % 16,
+	 * register r -> q = 4 * r + lane / 16 */
+	double *C = (double *)(uintptr_t)cb.c;
+	const bool bz = sc.flags & 2;
+	const bool none = pbeg == cb.pair_end;
+#pragma unroll
+	for (int i = 0; i < MT; i++)
+#pragma unroll
+		for (int j = 0; j < NT; j++)
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				uint32_t p = tl.p0 + wp + i * 16 + (lane & 15);
+				uint32_t q = tl.q0 + wq + j * 16 + 4 * r + (lane >> 4);
+				if (p < cb.p && q < cb.q) {
+					size_t o = (size_t)p + (size_t)q * cb.ldc;
+					double v;
+					if (bz)
+						v = none ? 0.0 : acc[i][j][r];
+					else if (none)
+						v = __dmul_rn(sc.beta_re, C[o]);
+					else
+						v = __dadd_rn(__dmul_rn(sc.beta_re, C[o]),
+						    acc[i][j][r]);
+					C[o] = v;
+				}
+			}
+}
+
+/* ---- FP32 / complex / reference kernel ---------------------------- */
+
+template <typename R> struct cx { R re, im; };
+
+template <typename T> struct ops;
+template <> struct ops<float> {
+	typedef float T;
+	static __device__ T zero() { return 0.f; }
+	static __device__ T mk(double re, double) { return (float)re; }
+	static __device__ T mul(T a, T b) { return a * b; }
+	static __device__ T fma(T a, T b, T c) { return __fmaf_rn(a, b, c); }
+	static __device__ T add(T a, T b) { return a + b; }
+};
+template <> struct ops<double> {
+	typedef double T;
+	static __device__ T zero() { return 0.0; }
+	static __device__ T mk(double re, double) { return re; }
+	static __device__ T mul(T a, T b) { return a * b; }
+	static __device__ T fma(T a, T b, T c) { return __fma_rn(a, b, c); }
+	static __device__ T add(T a, T b) { return a + b; }
+};
+template <typename R> struct ops<cx<R>> {
+	typedef cx<R> T;
+	static __device__ T zero() { return T{0, 0}; }
+	static __device__ T mk(double re, double im) { return T{(R)re, (R)im}; }
+	static __device__ T mul(T a, T b) {
+		return T{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+	}
+	static __device__ T fma(T a, T b, T c) {
+		c.re = ops<R>::fma(a.re, b.re, c.re);
+		c.re = ops<R>::fma(-a.im, b.im, c.re);
+		c.im = ops<R>::fma(a.re, b.im, c.im);
+		c.im = ops<R>::fma(a.im, b.re, c.im);
+		return c;
+	}
+	static __device__ T add(T a, T b) { return T{a.re + b.re, a.im + b.im}; }
+};
+
+/* 64 x 64 outputs per workgroup, 16 x 16 threads, 4 x 4 per thread. */
+template <typename T>
+__global__ void __launch_bounds__(GEMM_WG)
+k_gemm_simple(const gemm_tile *__restrict__ tiles,
+    const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
+    const gemm_scalars *__restrict__ scp)
+{
+	typedef ops<T> O;
+	constexpr int B = 64, BK = 16;
+	__shared__ T Xs[BK][B + 1];
+	__shared__ T Ys[BK][B + 1];
+	const gemm_tile tl = tiles[blockIdx.x];
+	const gemm_cblock cb = cbs[tl.cblock];
+	const gemm_scalars sc = *scp;
+	const int tid = threadIdx.x, tp = tid % 16, tq = tid / 16;
+	const T alpha = O::mk(sc.alpha_re, sc.alpha_im);
+	const T beta = O::mk(sc.beta_re, sc.beta_im);
+	T acc[4][4];
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++)
+			acc[i][j] = O::zero();
+	const uint32_t pbeg = (sc.flags & 1) ? cb.pair_end : cb.pair_begin;
+	for (uint32_t ip = pbeg; ip < cb.pair_end; ip++) {
+		const gemm_pair pr = pairs[ip];
+		const T al = O::mul(alpha, O::mk(pr.a_re, pr.a_im));
+		const T *X = (const T *)(uintptr_t)pr.x;
+		const T *Y = (const T *)(uintptr_t)pr.y;
+		for (uint32_t k0 = 0; k0 < pr.k; k0 += BK) {
+			__syncthreads();
+			for (int e = 0; e < B * BK / GEMM_WG; e++) {
+				int idx = tid + GEMM_WG * e, p, k;
+				if (pr.xmode == PANEL_PCONTIG) { p = idx % B; k = idx / B; }
+				else { k = idx % BK; p = idx / BK; }
+				uint32_t gp = tl.p0 + p, gk = k0 + k;
+				T v = O::zero();
+				if (gp < cb.p && gk < pr.k)
+					v = O::mul(X[pr.xmode == PANEL_PCONTIG ?
+					    (size_t)gp + (size_t)gk * pr.ldx :
+					    (size_t)gk + (size_t)gp * pr.ldx], al);
+				Xs[k][p] = v;
+				if (pr.ymode == PANEL_PCONTIG) { p = idx % B; k = idx / B; }
+				else { k = idx % BK; p = idx / BK; }
+				uint32_t gq = tl.q0 + p;
+				gk = k0 + k;
+				v = O::zero();
+				if (gq < cb.q && gk < pr.k)
+					v = Y[pr.ymode == PANEL_PCONTIG ?
+					    (size_t)gq + (size_t)gk * pr.ldy :
+					    (size_t)gk + (size_t)gq * pr.ldy];
+				Ys[k][p] = v;
+			}
+			__syncthreads();
+			for (int k = 0; k < BK; k++) {
+				T xa[4], yb[4];
+				for (int i = 0; i < 4; i++)
+					xa[i] = Xs[k][tp + 16 * i];
+				for (int j = 0; j < 4; j++)
+					yb[j] = Ys[k][tq + 16 * j];
+				for (int i = 0; i < 4; i++)
+					for (int j = 0; j < 4; j++)
+						acc[i][j] = O::fma(xa[i], yb[j], acc[i][j]);
+			}
+		}
+	}
+	T *C = (T *)(uintptr_t)cb.c;
+	const bool bz = sc.flags & 2, none = pbeg == cb.pair_end;
+	for (int i = 0; i < 4; i++)
+		for (int j = 0; j < 4; j++) {
+			uint32_t p = tl.p0 + tp + 16 * i, q = tl.q0 + tq + 16 * j;
+			if (p < cb.p && q < cb.q) {
+				size_t o = (size_t)p + (size_t)q * cb.ldc;
+				T v;
+				if (bz)
+					v = none ? O::zero() : acc[i][j];
+				else if (none)
+					v = O::mul(beta, C[o]);
+				else
+					v = O::add(O::mul(beta, C[o]), acc[i][j]);
+				C[o] = v;
+			}
+		}
+}
+
+/* ---- tiles and launch --------------------------------------------- */
+
+static int
+tile_edge(const contract_plan *pl)
+{
+	(void)pl;
+	return 64;
+}
+
+void
+build_tiles(contract_plan *pl)
+{
+	int B = tile_edge(pl);
+	pl->tile_cfg = B;
+	std::vector<std::pair<uint64_t, uint32_t>> order;
+	for (uint32_t i = 0; i < pl->cblocks.size(); i++) {
+		uint64_t kt = 0;
+		for (uint32_t p = pl->cblocks[i].pair_begin;
+		    p < pl->cblocks[i].pair_end; p++)
+			kt += pl->pairs[p].k;
+		order.push_back({kt, i});
+	}
+	/* longest K first (LPT) */
+	std::stable_sort(order.begin(), order.end(),
+	    [](const std::pair<uint64_t, uint32_t> &a,
+	    const std::pair<uint64_t, uint32_t> &b) { return a.first > b.first; });
+	pl->tiles.clear();
+	for (auto &o : order) {
+		const gemm_cblock &cb = pl->cblocks[o.second];
+		for (uint32_t q0 = 0; q0 < cb.q; q0 += B)
+			for (uint32_t p0 = 0; p0 < cb.p; p0 += B)
+				pl->tiles.push_back({o.second, p0, q0, 0});
+	}
+}
+
+static bool
+use_simple(void)
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("XM_GEMM_SIMPLE");
+		v = e && *e == '1';
+	}
+	return v;
+}
+
+void
+launch_gemm(contract_plan *pl, hipStream_t s)
+{
+	size_t nt = pl->tiles.size();
+	pl->gemm_launches = 0;
+	if (nt == 0)
+		return;
+	dim3 g((unsigned)nt), b(GEMM_WG);
+	switch (pl->type) {
+	case XM_SCALAR_DOUBLE:
+		if (use_simple())
+			k_gemm_simple<double><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		else
+			k_gemm_f64<64, 64><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		break;
+	case XM_SCALAR_FLOAT:
+		k_gemm_simple<float><<<g, b, 0, s>>>(pl->d_tiles, pl->d_cblocks,
+		    pl->d_pairs, pl->d_scalars);
+		break;
+	case XM_SCALAR_FLOAT_COMPLEX:
+		k_gemm_simple<cx<float>><<<g, b, 0, s>>>(pl->d_tiles,
+		    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		break;
+	case XM_SCALAR_DOUBLE_COMPLEX:
+		k_gemm_simple<cx<double>><<<g, b, 0, s>>>(pl->d_tiles,
+		    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		break;
+	}
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess)
+		xm_fatal("xm_contract: GEMM launch failed: %s",
+		    hipGetErrorString(e));
+	pl->gemm_launches = 1;
+}

@@ -1,0 +1,207 @@
+/*
+ * internal.h - shared declarations of the MI355X libxm implementation.
+ *
+ * Host side: block-space / tensor metadata (plain C++), the HBM arena and
+ * the device context (one HIP stream per process, default priority).
+ * Device side: kernels live in *.hip files and are launched through the
+ * launch_* functions declared at the bottom.
+ */
+#ifndef XM_INTERNAL_H
+#define XM_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "xm.h"
+#include "util.h"
+
+#define XM_API extern "C" __attribute__((visibility("default")))
+
+/* "libxm: <func>: <msg>" + exit(1), the error convention of libxm
+ * (libxm src/util.c:24-35); `fn` is the public function's name. */
+#define XM_FATAL(fn, msg) xm_fatal("%s: %s", (fn), (msg))
+
+/* Any HIP failure is fatal and names the public entry point. */
+#define HIP_OK(fn, call)                                                  \
+	do {                                                              \
+		hipError_t e_ = (call);                                   \
+		if (e_ != hipSuccess)                                     \
+			xm_fatal("%s: HIP error %s (%s) at %s:%d", (fn),  \
+			    hipGetErrorName(e_), hipGetErrorString(e_),   \
+			    __FILE__, __LINE__);                          \
+	} while (0)
+
+typedef std::complex<double> zdouble;
+
+/* ---- metadata ------------------------------------------------------ */
+
+struct xm_block_space {
+	xm_dim_t dims, nblocks;
+	std::vector<size_t> splits[XM_MAX_DIM];	/* nblocks+1 ascending */
+};
+
+struct xm_block {
+	xm_block_type_t type;
+	xm_dim_t perm;
+	xm_scalar_t scalar;
+	uint64_t data;	/* canonical: device address; derivative: flat index
+			   of the source block in the block grid */
+};
+
+struct xm_tensor {
+	xm_scalar_type_t type;
+	xm_block_space_t *bs;
+	xm_allocator_t *alloc;
+	std::vector<xm_block> blocks;
+	uint64_t id;		/* process-unique, never reused */
+	uint64_t version;	/* bumped on every structure change */
+};
+
+/* ---- HBM arena ----------------------------------------------------- */
+
+struct arena_slab {
+	char *base;
+	size_t bytes;
+	bool host;		/* pinned, device-mapped host memory */
+	std::vector<std::pair<size_t, size_t>> free_list; /* offset, size */
+};
+
+struct xm_allocator {
+	std::string path;
+	bool has_path;
+	std::mutex mu;
+	std::vector<arena_slab> slabs;
+	std::vector<std::pair<uint64_t, size_t>> live;	/* addr -> bytes */
+	size_t device_bytes, host_bytes, limit_bytes;
+	uint64_t id;
+};
+
+/* ---- helpers (host_meta.cpp) -------------------------------------- */
+
+size_t elem_size(xm_scalar_type_t t);
+bool is_complex(xm_scalar_type_t t);
+xm_block *tensor_block(const xm_tensor_t *t, const xm_dim_t &idx);
+size_t block_flat(const xm_tensor_t *t, const xm_dim_t &idx);
+xm_dim_t block_dims_flat(const xm_tensor_t *t, size_t flat);
+xm_dim_t grid_index(const xm_tensor_t *t, size_t flat);
+size_t block_elems(const xm_tensor_t *t, const xm_dim_t &idx);
+void bump_version(xm_tensor_t *t);
+/* Scalar arithmetic in the element precision (libxm src/scalar.c:45-95). */
+xm_scalar_t scalar_mul(xm_scalar_t a, xm_scalar_t b, xm_scalar_type_t t);
+xm_scalar_t scalar_add(xm_scalar_t a, xm_scalar_t b, xm_scalar_type_t t);
+
+/* Device address of the data of a (canonical or derivative) block. */
+uint64_t block_source_ptr(const xm_tensor_t *t, const xm_dim_t &idx);
+
+/* ---- device context (device.cpp) ----------------------------------- */
+
+hipStream_t xm_stream(void);
+/* Reusable device scratch of at least `bytes`, grown on demand; slot
+ * selects independent buffers.  Valid until the next call with the same
+ * slot. */
+void *scratch_buffer(int slot, size_t bytes);
+/* Upload a host vector into a device scratch slot (async on xm_stream). */
+void *upload(int slot, const void *host, size_t bytes);
+double elapsed_ms(hipEvent_t a, hipEvent_t b);
+
+/* ---- kernel launchers (*.hip) -------------------------------------- */
+
+/* A contiguous run of elements in device memory. */
+struct seg_desc {
+	uint64_t ptr;
+	uint64_t n;		/* elements */
+	uint64_t first_chunk;	/* prefix of chunk counts */
+	uint64_t aux;		/* per-kernel meaning */
+};
+
+void launch_fill_value(const seg_desc *dsegs, size_t nseg, size_t nchunks,
+    xm_scalar_type_t type, xm_scalar_t value, hipStream_t s);
+void launch_fill_random(const seg_desc *dsegs, size_t nseg, size_t nchunks,
+    xm_scalar_type_t type, uint64_t seed, double lo, double hi, double wlo,
+    double whi, hipStream_t s);
+
+/*
+ * Element-wise map between two blocks: for every element e of the
+ * destination block (canonical layout), the source element is at
+ * src + sum_d e_d * sstride[d] (strides already include the source
+ * block's permutation).  Operation codes are listed in kernels_elem.hip.
+ */
+#define XM_ELEM_MAXDIM 8
+struct elem_job {
+	uint64_t dst, src;
+	uint64_t n;			/* elements of the destination */
+	uint64_t first_chunk;
+	uint32_t ndim;
+	uint32_t src_zero;		/* source absent: treat as 0 */
+	uint32_t dims[XM_ELEM_MAXDIM];	/* destination extents */
+	uint64_t sstride[XM_ELEM_MAXDIM];
+	double f_re, f_im;		/* per-job factor */
+};
+
+enum elem_op {
+	ELEM_COPY_SCALED = 0,	/* dst = convert(f * src)          (xm_copy) */
+	ELEM_AXPY = 1,		/* dst = alpha*dst + f*src         (xm_add)  */
+	ELEM_MUL = 2,		/* dst = (f*dst)*src               (xm_mul)  */
+	ELEM_DIV = 3,		/* dst = dst/(f*src)               (xm_div)  */
+	ELEM_GATHER = 4,	/* dst = src (raw permuted copy)   (unfold)  */
+	ELEM_ACCUM = 5,		/* dst = dst + f*src               (adjoint fold) */
+};
+
+void launch_elem(const elem_job *djobs, size_t njobs, size_t nchunks,
+    int op, xm_scalar_type_t dtype, xm_scalar_type_t stype,
+    xm_scalar_t alpha, hipStream_t s);
+
+/* Per-block dot products reduced in a fixed order (xm_dot). */
+void launch_dot(const elem_job *djobs, size_t njobs, size_t nchunks,
+    xm_scalar_type_t type, double *dpartials, hipStream_t s);
+
+/* ---- grouped GEMM --------------------------------------------------- */
+
+/* Operand panel layout: element (p, k) of a P x K panel lives at
+ *   PCONTIG: p + k * ld      KCONTIG: k + p * ld                       */
+enum { PANEL_PCONTIG = 0, PANEL_KCONTIG = 1 };
+
+struct gemm_pair {
+	uint64_t x, y;		/* X panel (P x K), Y panel (Q x K) */
+	uint32_t k;		/* contracted extent */
+	uint8_t xmode, ymode;	/* PANEL_* */
+	uint16_t pad;
+	uint32_t ldx, ldy;
+	double a_re, a_im;	/* merged pair factor s_a*s_b */
+};
+
+struct gemm_cblock {
+	uint64_t c;		/* C panel: P x Q, element (p,q) at p + q*ldc */
+	uint32_t p, q, ldc;
+	uint32_t pair_begin, pair_end;
+	uint32_t flags;		/* bit0: C panel is a copy (not the block) */
+};
+
+struct gemm_tile {
+	uint32_t cblock;
+	uint32_t p0, q0;
+	uint32_t pad;
+};
+
+/* Scalars of one contraction call, read by the kernels from device
+ * memory so a captured launch sequence can be replayed. */
+struct gemm_scalars {
+	double alpha_re, alpha_im, beta_re, beta_im;
+	uint32_t flags;		/* bit0: alpha == 0, bit1: beta == 0 */
+	uint32_t pad;
+};
+
+/* Pack element jobs into their device form; launch from packed jobs
+ * already in device memory. */
+std::vector<char> elem_jobs_pack(const std::vector<elem_job> &jobs);
+void launch_elem_packed(const void *djobs, size_t njobs, size_t nchunks,
+    int op, xm_scalar_type_t dtype, xm_scalar_type_t stype,
+    xm_scalar_t alpha, hipStream_t s);
+
+#endif /* XM_INTERNAL_H */

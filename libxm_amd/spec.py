@@ -97,14 +97,21 @@ class Spec:
         return sum(self.block_size(i) for i in self.canonical_blocks()) * esz
 
     # ---- building in a library ------------------------------------------
-    def build(self, lib, alloc):
-        """Replay the recipe through lib's C API; returns the tensor."""
+    def build(self, lib, alloc, fake_ptrs=None):
+        """Replay the recipe through lib's C API; returns the tensor.
+        With fake_ptrs (an iterator of distinct integers) canonical blocks
+        get those data pointers through set_canonical_block_raw instead of
+        storage: enough for structure-only work, no data access allowed."""
         bs = lib.bs_create(self.dims, {k: p for k, p in enumerate(self.points)})
         t = lib.tensor(bs, self.type, alloc)
         lib.bs_free(bs)
         for op in self.ops:
             if op[0] == "c":
-                lib.set_canonical(t, op[1])
+                if fake_ptrs is None:
+                    lib.set_canonical(t, op[1])
+                else:
+                    lib.lib.xm_tensor_set_canonical_block_raw(
+                        t, xm.dim(op[1]), next(fake_ptrs))
             else:
                 lib.set_derivative(t, op[1], op[2], op[3], op[4])
         return t
