@@ -10,15 +10,16 @@ U[-1,1] (seed 0).  A "step" is one xm_contract call on tensors resident in
 HBM.
 
 FLOPs F = sum over merged block pairs with nonzero factor of 2*m*n*k (the
-GEMM work libxm itself executes).  value = F / (wall time per step).
-roofline.achieved = F / (average device time of the grouped GEMM kernel,
-HIP events on the library's stream), against 78.6 TFLOP/s FP64 (or the BF16
-matrix peak, for the split form).
+GEMM work libxm itself executes).  value = F / (wall time per step) of the
+FP64 path (the drop-in); the split-precision (BF16 slices) form is reported
+beside it with the same F, its error against FP64 results, and its BF16
+matrix-core utilisation.  roofline.achieved = F / (average device time of
+the grouped GEMM kernel, HIP events on the library's stream).
 
 The CPU baseline runs libxm itself (compiled from its sources into
 oracle/_ref, sequential MKL per OpenMP thread) on a bounded sample of the
-same workload: the canonical C blocks of the first few (a,b) virtual block
-pairs, timed on the host cores, scaled as TFLOP/s of the sample's own F.
+same workload, the canonical C blocks of the first (a,b) virtual block
+pairs, timed on the host cores as TFLOP/s of the sample's own F.
 """
 import argparse
 import json
@@ -38,14 +39,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def products(nslices):
+    """BF16 slice products per FP64 product: pairs (t, u), t + u <= s + 1."""
+    return nslices * (nslices + 1) // 2
+
+
+def fill_tensor(lib, w, x, seed):
+    from libxm_amd import workloads
+    lib.fill_random(x, seed)
+    for ia, ib in workloads.antisym_strings(4, w.pairs):
+        lib.add(0.5, x, -0.5, x, ia, ib)
+
+
 def build_workload(lib, w, seed=0):
     alloc = lib.allocator()
     t, v, c = (s.build(lib, alloc) for s in w.specs)
     for k, x in enumerate((t, v, c)):
-        lib.fill_random(x, seed * 16 + k)
-        for ia, ib in __import__("libxm_amd.workloads",
-                                 fromlist=["x"]).antisym_strings(4, w.pairs):
-            lib.add(0.5, x, -0.5, x, ia, ib)
+        fill_tensor(lib, w, x, seed * 16 + k)
     lib.synchronize()
     return alloc, t, v, c
 
@@ -54,28 +64,22 @@ def time_steps(lib, w, t, v, c, steps, warmup):
     for _ in range(warmup):
         lib.contract(w.alpha, t, v, w.beta, c, *w.idx)
     lib.synchronize()
-    gemm = []
+    gemm, prep = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         lib.contract(w.alpha, t, v, w.beta, c, *w.idx)
-        gemm.append(lib.stats()["gemm_ms"])
+        st = lib.stats()
+        gemm.append(st["gemm_ms"])
+        prep.append(st["prep_ms"])
     lib.synchronize()
     dt = time.perf_counter() - t0
-    return dt / steps * 1e3, sum(gemm) / len(gemm), lib.stats()
+    return (dt / steps * 1e3, sum(gemm) / len(gemm), sum(prep) / len(prep),
+            lib.stats())
 
 
-def cpu_baseline(lib, w, alloc_gpu, t, v, c, budget_s=20.0):
-    """libxm (or, if it is absent, the oracle) on a bounded sample."""
-    from libxm_amd import xm
-    from libxm_amd.spec import Spec, rel_err
-    ref_so = os.path.join(ROOT, "oracle", "_ref", "libxmref.so")
-    port_so = os.path.join(ROOT, "oracle", "liboracle.so")
-    kind = "reference" if os.path.exists(ref_so) else "port"
-    os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
-    cpu = xm.XmLib(ref_so if kind == "reference" else port_so)
-    oracle = xm.XmLib(port_so)
-    ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    # sample: canonical C blocks of the first (a,b) virtual block pairs
+def sample_blocks(lib, w, t, v, c):
+    """Canonical C blocks of the first (a,b) virtual block pairs, about half
+    of the workload's FLOPs, and the a/b source blocks they read."""
     cblocks = w.c.canonical_blocks()
     abs_ = []
     for idx in cblocks:
@@ -85,8 +89,7 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, budget_s=20.0):
     f_of = {}
     for p in pairs_all:
         f_of[p["c"]] = f_of.get(p["c"], 0) + 2.0 * p["m"] * p["n"] * p["k"]
-    # ~budget_s at an assumed 0.3 TFLOP/s
-    target = 0.3e12 * budget_s
+    target = 0.5 * sum(f_of.values())
     chosen, fsum = [], 0.0
     for ab in abs_:
         blocks = [x for x in cblocks if x[2:] == ab]
@@ -94,17 +97,32 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, budget_s=20.0):
             break
         chosen += blocks
         fsum += sum(f_of.get(x, 0) for x in blocks)
-    cs = Spec(w.c.dims, {k: p for k, p in enumerate(w.c.points)}, w.c.type)
-    for idx in chosen:
-        cs.canon(idx)
-    need_a = {p["a"] for p in pairs_all if p["c"] in set(chosen)}
-    need_b = {p["b"] for p in pairs_all if p["c"] in set(chosen)}
+    sel = set(chosen)
 
     def src(spec, idx):
         k = spec.kind(idx)
         return idx if k == "c" else k[1]
-    need_a = {src(w.t, i) for i in need_a}
-    need_b = {src(w.v, i) for i in need_b}
+    need_a = {src(w.t, p["a"]) for p in pairs_all if p["c"] in sel}
+    need_b = {src(w.v, p["b"]) for p in pairs_all if p["c"] in sel}
+    return chosen, fsum, need_a, need_b, len(cblocks)
+
+
+def cpu_baseline(lib, w, alloc_gpu, t, v, c, split_slices=None):
+    """libxm (or, if it is absent, the oracle) on a bounded sample; the same
+    sample is recomputed on the GPU (FP64 and split) for parity."""
+    from libxm_amd import xm
+    from libxm_amd.spec import Spec, rel_err
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libxmref.so")
+    port_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    kind = "reference" if os.path.exists(ref_so) else "port"
+    os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+    cpu = xm.XmLib(ref_so if kind == "reference" else port_so)
+    oracle = xm.XmLib(port_so)
+    ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    chosen, fsum, need_a, need_b, ncb = sample_blocks(lib, w, t, v, c)
+    cs = Spec(w.c.dims, {k: p for k, p in enumerate(w.c.points)}, w.c.type)
+    for idx in chosen:
+        cs.canon(idx)
     alloc = cpu.allocator()
     ct, cv = w.t.build(cpu, alloc), w.v.build(cpu, alloc)
     cc = cs.build(cpu, alloc)
@@ -113,22 +131,26 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, budget_s=20.0):
     for i in need_b:
         cpu.write_block(cv, i, lib.read_block(v, i))
     c0 = {i: lib.read_block(c, i) for i in chosen}
-    for i in chosen:
-        cpu.write_block(cc, i, c0[i])
-    cpu.contract(w.alpha, ct, cv, w.beta, cc, *w.idx)      # warm
-    for i in chosen:
-        cpu.write_block(cc, i, c0[i])
-    t0 = time.perf_counter()
-    cpu.contract(w.alpha, ct, cv, w.beta, cc, *w.idx)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(3):
+        for i in chosen:
+            cpu.write_block(cc, i, c0[i])
+        t0 = time.perf_counter()
+        cpu.contract(w.alpha, ct, cv, w.beta, cc, *w.idx)
+        times.append(time.perf_counter() - t0)
+    dt = sorted(times)[1]
     cpu_out = {i: cpu.read_block(cc, i) for i in chosen}
-    # same sample on the GPU: parity of the timed CPU result with ours
+    errs = {}
     gc = cs.build(lib, alloc_gpu)
-    for i in chosen:
-        lib.write_block(gc, i, c0[i])
-    lib.contract(w.alpha, t, v, w.beta, gc, *w.idx)
-    gpu_out = {i: lib.read_block(gc, i) for i in chosen}
-    err = rel_err(gpu_out, cpu_out)
+    for prec, ns in [(xm.PRECISION_NATIVE, None)] + (
+            [(xm.PRECISION_SPLIT_BF16, split_slices)] if split_slices else []):
+        lib.set_precision(prec, ns)
+        for i in chosen:
+            lib.write_block(gc, i, c0[i])
+        lib.contract(w.alpha, t, v, w.beta, gc, *w.idx)
+        gpu_out = {i: lib.read_block(gc, i) for i in chosen}
+        errs[prec] = rel_err(gpu_out, cpu_out)
+    lib.set_precision(xm.PRECISION_NATIVE)
     lib.tensor_free(gc)
     for x in (ct, cv, cc):
         cpu.tensor_free(x)
@@ -136,11 +158,59 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, budget_s=20.0):
     blas = oracle.lib.xm_oracle_blas_name().decode()
     return {"value": fsum / dt / 1e12, "unit": "TFLOP/s", "cores": ncores,
             "kind": kind,
-            "sample": "%d of %d canonical C blocks (first %d (a,b) virtual "
-                      "block pairs), %.3g GFLOP, %.2f s; BLAS %s" % (
-                          len(chosen), len(cblocks), len(
-                              {x[2:] for x in chosen}), fsum / 1e9, dt, blas),
-            "rel_err_vs_gpu": err}
+            "sample": "%d of %d canonical C blocks (the first %d (a,b) "
+                      "virtual block pairs), %.3g GFLOP, median of 3 runs "
+                      "%.2f s, OpenMP %d threads, BLAS %s" % (
+                          len(chosen), ncb, len({x[2:] for x in chosen}),
+                          fsum / 1e9, dt, ncores, blas),
+            "rel_err_fp64_gpu_vs_cpu": errs[xm.PRECISION_NATIVE],
+            "rel_err_split_gpu_vs_cpu": errs.get(xm.PRECISION_SPLIT_BF16)}
+
+
+def split_accuracy(lib, w, alloc, t, v, nslices):
+    """Product term alone (beta = 0): split vs FP64 on the whole workload."""
+    from libxm_amd import xm
+    from libxm_amd.spec import rel_err
+    c1, c2 = w.c.build(lib, alloc), w.c.build(lib, alloc)
+    lib.set_precision(xm.PRECISION_NATIVE)
+    lib.contract(w.alpha, t, v, 0.0, c1, *w.idx)
+    lib.set_precision(xm.PRECISION_SPLIT_BF16, nslices)
+    lib.contract(w.alpha, t, v, 0.0, c2, *w.idx)
+    lib.set_precision(xm.PRECISION_NATIVE)
+    err = rel_err(w.c.read(lib, c2), w.c.read(lib, c1))
+    lib.tensor_free(c1)
+    lib.tensor_free(c2)
+    return err
+
+
+def copy_calibration(lib, steps):
+    """xm_copy(b, 1, a, "ij", "ij") of 2^29 doubles in 64 blocks: every
+    launch reads 4 GiB and writes 4 GiB."""
+    from libxm_amd import xm
+    from libxm_amd.spec import Spec
+    sp = Spec((1 << 15, 1 << 14), {0: [k << 9 for k in range(1, 64)]},
+              xm.SCALAR_DOUBLE).all_canonical()
+    alloc = lib.allocator()
+    a, b = sp.build(lib, alloc), sp.build(lib, alloc)
+    lib.fill_random(a, 1)
+    dt = 0.0
+    for _ in range(max(steps, 1)):
+        t0 = time.perf_counter()
+        lib.copy(b, 1.0, a, "ij", "ij")
+        dt = time.perf_counter() - t0
+    print(json.dumps({"mode": "copy", "bytes_read": 8 << 29,
+                      "bytes_written": 8 << 29, "seconds": dt,
+                      "GBps": 2 * (8 << 29) / dt / 1e9}), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary of
+    this workload (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        return json.load(open(path)).get(kernel)
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -151,8 +221,12 @@ def main():
     ap.add_argument("--o", type=int, default=64)
     ap.add_argument("--v", type=int, default=384)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--slices", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--split", choices=["auto", "off", "only"], default="auto")
+    ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--mode", choices=["contract", "copy"], default="contract",
+                    help="copy: xm_copy of a 4 GiB FP64 tensor, a plain "
+                         "copy of known size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build runs on a single "
@@ -161,6 +235,8 @@ def main():
 
     from libxm_amd import xm, workloads
     lib = xm.load()
+    if args.mode == "copy":
+        return copy_calibration(lib, args.steps)
     w = workloads.ladder_d2h(args.o, args.v, args.seed)
     log("workload", json.dumps(w.describe()))
     tb = time.perf_counter()
@@ -170,7 +246,8 @@ def main():
                                                    time.perf_counter() - tb))
 
     lib.set_precision(xm.PRECISION_NATIVE)
-    ms, gemm_ms, st = time_steps(lib, w, t, v, c, args.steps, args.warmup)
+    ms, gemm_ms, prep_ms, st = time_steps(lib, w, t, v, c, args.steps,
+                                          args.warmup)
     F = st["flops"]
     log("fp64: %.3f ms/step, gemm %.3f ms, %.2f TFLOP/s" % (
         ms, gemm_ms, F / ms / 1e9))
@@ -195,12 +272,36 @@ def main():
         "roofline": {"bound": "mfma", "achieved": F / (gemm_ms * 1e-3) / 1e12,
                      "peak": FP64_PEAK, "unit": "TFLOP/s",
                      "frac": F / (gemm_ms * 1e-3) / 1e12 / FP64_PEAK,
-                     "traffic": None, "kernel": "k_gemm_f64",
-                     "kernel_ms": gemm_ms},
+                     "traffic": measured_traffic("k_gemm_f64"),
+                     "kernel": "k_gemm_f64", "kernel_ms": gemm_ms},
     }
+    if not args.no_split:
+        lib.set_precision(xm.PRECISION_SPLIT_BF16, args.slices)
+        sms, sgemm, sprep, sst = time_steps(lib, w, t, v, c, args.steps,
+                                            args.warmup)
+        lib.set_precision(xm.PRECISION_NATIVE)
+        nprod = products(args.slices)
+        err = split_accuracy(lib, w, alloc, t, v, args.slices)
+        log("split: %.3f ms/step (slice %.3f, gemm %.3f), %.2f TFLOP/s, "
+            "rel err %.2e" % (sms, sprep, sgemm, F / sms / 1e9, err))
+        result["split_bf16"] = {
+            "value": F / (sms * 1e-3) / 1e12,
+            "unit": "TFLOP/s (FP64-equivalent, same F)",
+            "ms_per_step": sms, "slice_ms": sprep, "gemm_ms": sgemm,
+            "nslices": args.slices, "bf16_products_per_fp64_product": nprod,
+            "speedup_vs_fp64": ms / sms,
+            "rel_err_vs_fp64_gpu": err, "meets_1e-12": err <= 1e-12,
+            "roofline": {"bound": "mfma",
+                         "achieved": F * nprod / (sgemm * 1e-3) / 1e12,
+                         "peak": BF16_PEAK, "unit": "TFLOP/s (BF16 MFMA)",
+                         "frac": F * nprod / (sgemm * 1e-3) / 1e12 / BF16_PEAK,
+                         "traffic": measured_traffic("k_gemm_split"),
+                         "kernel": "k_gemm_split", "kernel_ms": sgemm}}
     if not args.no_cpu:
         try:
-            result["cpu_baseline"] = cpu_baseline(lib, w, alloc, t, v, c)
+            result["cpu_baseline"] = cpu_baseline(
+                lib, w, alloc, t, v, c,
+                None if args.no_split else args.slices)
         except Exception as e:  # reported, never silently replaced
             result["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(result), flush=True)

@@ -674,9 +674,9 @@ get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 	*cached = false;
 	contract_plan *pl = build_plan(a, b, c, mk, prec);
 	pl->nslices = key.nslices;
+	plan_upload(pl);
 	if (prec == XM_PRECISION_SPLIT_BF16)
 		split_prepare(pl, a, b, c, mk);
-	plan_upload(pl);
 	cache.push_front({key, pl});
 	while (cache.size() > CACHE_MAX) {
 		plan_release(cache.back().second);

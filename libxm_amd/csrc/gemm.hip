@@ -51,6 +51,8 @@ k_gemm_f64(const gemm_tile *__restrict__ tiles,
 	__shared__ double Ys[BK * LQ];
 
 	const gemm_tile tl = tiles[blockIdx.x];
+	if (tl.cblock == 0xffffffffu)
+		return;
 	const gemm_cblock cb = cbs[tl.cblock];
 	const gemm_scalars sc = *scp;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -211,6 +213,8 @@ k_gemm_simple(const gemm_tile *__restrict__ tiles,
 	__shared__ T Xs[BK][B + 1];
 	__shared__ T Ys[BK][B + 1];
 	const gemm_tile tl = tiles[blockIdx.x];
+	if (tl.cblock == 0xffffffffu)
+		return;
 	const gemm_cblock cb = cbs[tl.cblock];
 	const gemm_scalars sc = *scp;
 	const int tid = threadIdx.x, tp = tid % 16, tq = tid / 16;
@@ -291,30 +295,50 @@ tile_edge(const contract_plan *pl)
 	return 64;
 }
 
+/*
+ * Work tiles in launch order.  Workgroup w of a launch runs on XCD w % 8
+ * (round-robin dispatch) and each XCD has its own L2, so the tiles of one
+ * C block are kept on one XCD: C blocks are spread over the 8 XCDs by a
+ * greedy longest-first balance of their work, every XCD walks its C blocks
+ * with the P-tiles innermost (tiles running together share the Y panel
+ * columns and the X panel), and slot s of XCD x becomes workgroup
+ * 8 * s + x.  Shorter XCD lists are padded with empty tiles.
+ */
 void
 build_tiles(contract_plan *pl)
 {
+	const int NXCD = 8;
 	int B = tile_edge(pl);
 	pl->tile_cfg = B;
-	std::vector<std::pair<uint64_t, uint32_t>> order;
+	std::vector<std::pair<double, uint32_t>> order;
 	for (uint32_t i = 0; i < pl->cblocks.size(); i++) {
-		uint64_t kt = 0;
-		for (uint32_t p = pl->cblocks[i].pair_begin;
-		    p < pl->cblocks[i].pair_end; p++)
+		const gemm_cblock &cb = pl->cblocks[i];
+		double kt = 1;
+		for (uint32_t p = cb.pair_begin; p < cb.pair_end; p++)
 			kt += pl->pairs[p].k;
-		order.push_back({kt, i});
+		double nt = (double)((cb.p + B - 1) / B) * ((cb.q + B - 1) / B);
+		order.push_back({kt * nt, i});
 	}
-	/* longest K first (LPT) */
 	std::stable_sort(order.begin(), order.end(),
-	    [](const std::pair<uint64_t, uint32_t> &a,
-	    const std::pair<uint64_t, uint32_t> &b) { return a.first > b.first; });
-	pl->tiles.clear();
+	    [](const std::pair<double, uint32_t> &a,
+	    const std::pair<double, uint32_t> &b) { return a.first > b.first; });
+	std::vector<gemm_tile> bins[NXCD];
+	double load[NXCD] = {0};
 	for (auto &o : order) {
+		int x = (int)(std::min_element(load, load + NXCD) - load);
+		load[x] += o.first;
 		const gemm_cblock &cb = pl->cblocks[o.second];
 		for (uint32_t q0 = 0; q0 < cb.q; q0 += B)
 			for (uint32_t p0 = 0; p0 < cb.p; p0 += B)
-				pl->tiles.push_back({o.second, p0, q0, 0});
+				bins[x].push_back({o.second, p0, q0, 0});
 	}
+	size_t per = 0;
+	for (auto &b : bins)
+		per = std::max(per, b.size());
+	pl->tiles.assign(per * NXCD, gemm_tile{0xffffffffu, 0, 0, 0});
+	for (int x = 0; x < NXCD; x++)
+		for (size_t s = 0; s < bins[x].size(); s++)
+			pl->tiles[s * NXCD + x] = bins[x][s];
 }
 
 static bool
