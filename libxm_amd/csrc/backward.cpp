@@ -1,12 +1,163 @@
 /*
- * backward.cpp - xm_contract_backward (no libxm counterpart).
+ * backward.cpp - xm_contract_backward, the adjoint of xm_contract
+ * (no libxm counterpart).
+ *
+ * Forward: c = alpha * a * b + beta * c over the canonical blocks of c
+ * (libxm src/contract.c:117-225).  Loss: L = sum over the canonical
+ * entries of c of dc * c.  Every nonzero block X of a (canonical or
+ * derivative) enters the forward as X(e) = s_X * S(permute(e, p_X)), so
+ *
+ *   G_a(X) = alpha * sum_{canonical C, k} dc * b     (one GEMM per pair),
+ *   da[S](permute(e, p_X)) += s_X * G_a(X)(e)        for every X with
+ *                                                     source S.
+ *
+ * G_a is one more contraction through the same planner and grouped GEMM:
+ * xm_contract(alpha, dc', b, 0, G_a, idxc, idxb, idxa), where dc' views the
+ * canonical blocks of dc (its derivative blocks count as zero) and G_a has
+ * the nonzero blocks of a as independent canonical blocks.  Canonical
+ * blocks of G_a are da's own storage; derivative blocks of a get scratch
+ * storage and are then folded into their sources by the permute kernel,
+ * in launch waves that never write one source block twice.
+ * db is computed the same way from xm_contract(alpha, a, dc', 0, G_b,
+ * idxa, idxc, idxb).
  */
 #include "contract_plan.h"
 
-XM_API void
-xm_contract_backward(xm_scalar_t, const xm_tensor_t *, const xm_tensor_t *,
-    const xm_tensor_t *, xm_tensor_t *, xm_tensor_t *, const char *,
-    const char *, const char *)
+#include <chrono>
+#include <map>
+
+static const char *FN = "xm_contract_backward";
+
+/* A view of dc holding only its canonical blocks (no data copied). */
+static xm_tensor_t *
+canonical_view(const xm_tensor_t *dc)
 {
-	XM_FATAL("xm_contract_backward", "not available");
+	xm_tensor_t *v = xm_tensor_create(dc->bs, dc->type, dc->alloc);
+	for (size_t f = 0; f < dc->blocks.size(); f++)
+		if (dc->blocks[f].type == XM_BLOCK_TYPE_CANONICAL)
+			xm_tensor_set_canonical_block_raw(v, grid_index(dc, f),
+			    dc->blocks[f].data);
+	return v;
+}
+
+/* One gradient: g = every nonzero block of x made canonical; canonical
+ * blocks share dx's storage. */
+static void
+gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
+    const xm_tensor_t *p, const xm_tensor_t *q, const char *ip,
+    const char *iq, const char *ix, bool x_first, xm_contract_stats_t *acc)
+{
+	if (dx->type != x->type)
+		XM_FATAL(FN, "gradient tensors must have the tensor's scalar type");
+	if (!xm_block_space_eq(dx->bs, x->bs))
+		XM_FATAL(FN, "gradient tensors must have the tensor's block space");
+	xm_tensor_t *g = xm_tensor_create(x->bs, x->type, x->alloc);
+	std::vector<uint64_t> scratch;
+	for (size_t f = 0; f < x->blocks.size(); f++) {
+		const xm_block &bx = x->blocks[f];
+		if (bx.type == XM_BLOCK_TYPE_ZERO)
+			continue;
+		xm_dim_t idx = grid_index(x, f);
+		if (bx.type == XM_BLOCK_TYPE_CANONICAL) {
+			if (dx->blocks[f].type != XM_BLOCK_TYPE_CANONICAL)
+				XM_FATAL(FN, "gradient tensors must have canonical "
+				    "blocks where the tensor has them");
+			xm_tensor_set_canonical_block_raw(g, idx,
+			    dx->blocks[f].data);
+		} else {
+			xm_tensor_set_canonical_block(g, idx);
+			scratch.push_back(g->blocks[f].data);
+		}
+	}
+	xm_contract(alpha, p, q, 0, g, ip, iq, ix);
+	xm_contract_stats_t st;
+	xm_contract_get_stats(&st);
+	acc->flops += st.flops;
+	acc->flops_pre += st.flops_pre;
+	acc->npairs += st.npairs;
+	acc->npairs_pre += st.npairs_pre;
+	acc->nblocks_c += st.nblocks_c;
+	acc->nunfold += st.nunfold;
+	acc->unfold_bytes += st.unfold_bytes;
+	acc->gemm_ms += st.gemm_ms;
+	acc->prep_ms += st.prep_ms;
+	acc->post_ms += st.post_ms;
+	acc->gemm_launches += st.gemm_launches;
+	acc->precision = st.precision;
+	acc->nslices = st.nslices;
+	(void)x_first;
+
+	/* fold derivative blocks into their sources:
+	 * dx[S](permute(e, p)) += s * g[X](e) */
+	std::map<size_t, int> waves_of;	/* source -> jobs so far */
+	std::vector<std::vector<elem_job>> waves;
+	for (size_t f = 0; f < x->blocks.size(); f++) {
+		const xm_block &bx = x->blocks[f];
+		if (bx.type != XM_BLOCK_TYPE_DERIVATIVE)
+			continue;
+		size_t src = (size_t)bx.data;
+		xm_dim_t sd = block_dims_flat(x, src);
+		xm_dim_t xd = block_dims_flat(x, f);
+		elem_job j;
+		memset(&j, 0, sizeof j);
+		j.dst = dx->blocks[src].data;
+		j.src = g->blocks[f].data;
+		j.n = xm_dim_dot(&sd);
+		j.ndim = (uint32_t)sd.n;
+		uint64_t xs[XM_MAX_DIM], s = 1;
+		for (size_t k = 0; k < xd.n; k++) {
+			xs[k] = s;
+			s *= xd.i[k];
+		}
+		/* source dim perm[k] is logical dim k of X */
+		for (size_t k = 0; k < xd.n; k++) {
+			j.dims[bx.perm.i[k]] = (uint32_t)sd.i[bx.perm.i[k]];
+			j.sstride[bx.perm.i[k]] = xs[k];
+		}
+		j.f_re = bx.scalar.real();
+		j.f_im = bx.scalar.imag();
+		int w = waves_of[src]++;
+		if ((int)waves.size() <= w)
+			waves.resize(w + 1);
+		if (j.n)
+			waves[w].push_back(j);
+	}
+	hipStream_t s = xm_stream();
+	for (auto &wv : waves) {
+		size_t c = 0;
+		for (auto &j : wv) {
+			j.first_chunk = c;
+			c += (j.n + 4095) / 4096;
+		}
+		launch_elem(wv.data(), wv.size(), c, ELEM_ACCUM, x->type, x->type,
+		    0, s);
+		HIP_OK(FN, hipStreamSynchronize(s));
+	}
+	for (uint64_t ptr : scratch)
+		xm_allocator_deallocate(x->alloc, ptr);
+	xm_tensor_free(g);	/* metadata only; storage released above */
+}
+
+XM_API void
+xm_contract_backward(xm_scalar_t alpha, const xm_tensor_t *a,
+    const xm_tensor_t *b, const xm_tensor_t *dc, xm_tensor_t *da,
+    xm_tensor_t *db, const char *idxa, const char *idxb, const char *idxc)
+{
+	auto t0 = std::chrono::steady_clock::now();
+	masks_t mk;
+	/* the forward's own checks (libxm src/contract.c:237-280) */
+	contract_masks(FN, a, b, dc, idxa, idxb, idxc, mk);
+	if ((da && da->alloc != a->alloc) || (db && db->alloc != b->alloc))
+		XM_FATAL(FN, "tensors must use same allocator");
+	xm_contract_stats_t acc;
+	memset(&acc, 0, sizeof acc);
+	xm_tensor_t *dcv = canonical_view(dc);
+	if (da)
+		gradient(alpha, a, da, dcv, b, idxc, idxb, idxa, true, &acc);
+	if (db)
+		gradient(alpha, b, db, a, dcv, idxa, idxc, idxb, false, &acc);
+	xm_tensor_free(dcv);
+	auto t1 = std::chrono::steady_clock::now();
+	acc.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+	contract_set_stats(acc);
 }

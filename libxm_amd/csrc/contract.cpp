@@ -685,8 +685,6 @@ get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 	return pl;
 }
 
-/* Drop every cached plan that references tensor id `id` (called when a
- * tensor is freed so its storage can be reused safely). */
 void
 plan_cache_clear(void)
 {
@@ -694,6 +692,23 @@ plan_cache_clear(void)
 	for (auto &p : cache)
 		plan_release(p.second);
 	cache.clear();
+}
+
+/* Drop the cached plans that reference tensor `id` (the tensor is being
+ * freed: its plans can never be reused and hold device memory). */
+void
+plan_cache_drop(uint64_t id)
+{
+	std::lock_guard<std::mutex> g(cache_mu);
+	for (auto it = cache.begin(); it != cache.end();) {
+		const plan_key &k = it->first;
+		if (k.ida == id || k.idb == id || k.idc == id) {
+			plan_release(it->second);
+			it = cache.erase(it);
+		} else {
+			++it;
+		}
+	}
 }
 
 /* ---- execution ------------------------------------------------------ */
@@ -774,6 +789,12 @@ XM_API void
 xm_contract_get_stats(xm_contract_stats_t *s)
 {
 	*s = last_stats;
+}
+
+void
+contract_set_stats(const xm_contract_stats_t &s)
+{
+	last_stats = s;
 }
 
 XM_API size_t

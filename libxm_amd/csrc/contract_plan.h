@@ -77,6 +77,8 @@ size_t enumerate_pairs(const xm_tensor_t *a, const xm_tensor_t *b,
 int current_precision(void);
 int current_nslices(void);
 void plan_cache_clear(void);
+void plan_cache_drop(uint64_t tensor_id);
+void contract_set_stats(const xm_contract_stats_t &s);
 
 /* gemm.hip */
 void build_tiles(contract_plan *pl);
