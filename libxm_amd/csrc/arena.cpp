@@ -151,6 +151,7 @@ xm_allocator_deallocate(xm_allocator_t *a, uint64_t p)
 {
 	if (p == XM_NULL_PTR)
 		return;
+	bump_data_epoch();
 	std::lock_guard<std::mutex> g(a->mu);
 	auto it = std::lower_bound(a->live.begin(), a->live.end(),
 	    std::make_pair(p, (size_t)0));
@@ -206,6 +207,7 @@ xm_allocator_write(xm_allocator_t *a, uint64_t p, const void *mem,
     size_t bytes)
 {
 	check_range("xm_allocator_write", a, p, bytes);
+	bump_data_epoch();
 	HIP_OK("xm_allocator_write", hipMemcpyAsync((void *)(uintptr_t)p, mem,
 	    bytes, hipMemcpyHostToDevice, xm_stream()));
 	HIP_OK("xm_allocator_write", hipStreamSynchronize(xm_stream()));

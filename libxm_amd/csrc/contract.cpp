@@ -720,6 +720,7 @@ run_plan(contract_plan *pl, xm_scalar_t alpha, xm_scalar_t beta)
 {
 	const char *fn = "xm_contract";
 	hipStream_t s = xm_stream();
+	bump_data_epoch();
 	gemm_scalars sc;
 	sc.alpha_re = alpha.real();
 	sc.alpha_im = alpha.imag();

@@ -7,6 +7,7 @@
 #include "internal.h"
 
 #include <atomic>
+#include <map>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -965,8 +966,30 @@ xm_tensor_write_block(xm_tensor_t *t, xm_dim_t idx, const void *buf)
 	    xm_tensor_get_block_bytes(t, idx));
 }
 
-/* One element: only that element is copied from the device
- * (libxm src/tensor.c:154-178 reads the whole block). */
+/*
+ * Device data epoch: bumped by every entry point that writes block storage
+ * (copies into blocks, kernels, deallocation).  Host copies of blocks read
+ * by xm_tensor_get_element stay valid while the epoch is unchanged.
+ */
+static std::atomic<uint64_t> data_epoch{1};
+
+void
+bump_data_epoch(void)
+{
+	data_epoch++;
+}
+
+struct element_cache {
+	std::mutex mu;
+	uint64_t epoch = 0;
+	size_t bytes = 0;
+	std::map<uint64_t, std::vector<unsigned char>> blocks; /* by address */
+};
+static element_cache ecache;
+
+/* One element (libxm src/tensor.c:154-178).  The block's storage is copied
+ * to the host once per data epoch and kept, so element-by-element loops
+ * over a tensor cost one device copy per block, not one per element. */
 XM_API xm_scalar_t
 xm_tensor_get_element(const xm_tensor_t *t, xm_dim_t idx)
 {
@@ -979,10 +1002,30 @@ xm_tensor_get_element(const xm_tensor_t *t, xm_dim_t idx)
 	xm_dim_t pe = xm_dim_permute(&ei, &b->perm);
 	xm_dim_t pd = xm_dim_permute(&bd, &b->perm);
 	size_t off = xm_dim_offset(&pe, &pd), es = elem_size(t->type);
+	size_t bytes = xm_dim_dot(&bd) * es;
+	uint64_t src = block_source_ptr(t, bi);
 	unsigned char v[16];
-	HIP_OK("xm_tensor_get_element", hipMemcpy(v,
-	    (const char *)(uintptr_t)block_source_ptr(t, bi) + off * es, es,
-	    hipMemcpyDeviceToHost));
+	{
+		std::lock_guard<std::mutex> g(ecache.mu);
+		uint64_t ep = data_epoch.load();
+		if (ecache.epoch != ep || ecache.bytes > (256u << 20)) {
+			ecache.blocks.clear();
+			ecache.bytes = 0;
+			ecache.epoch = ep;
+		}
+		auto it = ecache.blocks.find(src);
+		if (it == ecache.blocks.end() || it->second.size() < bytes) {
+			std::vector<unsigned char> h(bytes);
+			HIP_OK("xm_tensor_get_element", hipMemcpyAsync(h.data(),
+			    (const void *)(uintptr_t)src, bytes,
+			    hipMemcpyDeviceToHost, xm_stream()));
+			HIP_OK("xm_tensor_get_element",
+			    hipStreamSynchronize(xm_stream()));
+			ecache.bytes += bytes;
+			it = ecache.blocks.insert_or_assign(src, std::move(h)).first;
+		}
+		memcpy(v, it->second.data() + off * es, es);
+	}
 	return scalar_mul(b->scalar, xm_scalar_get_element(v, 0, t->type),
 	    t->type);
 }

@@ -98,6 +98,9 @@ xm_scalar_t scalar_add(xm_scalar_t a, xm_scalar_t b, xm_scalar_type_t t);
 
 /* Device address of the data of a (canonical or derivative) block. */
 uint64_t block_source_ptr(const xm_tensor_t *t, const xm_dim_t &idx);
+/* Invalidate host copies of block data (called before any device write to
+ * block storage). */
+void bump_data_epoch(void);
 
 /* ---- device context (device.cpp) ----------------------------------- */
 

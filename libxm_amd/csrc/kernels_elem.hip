@@ -336,6 +336,7 @@ launch_fill_value(const seg_desc *dsegs, size_t nseg, size_t nchunks,
 {
 	if (nchunks == 0)
 		return;
+	bump_data_epoch();
 	dim3 g((unsigned)nchunks), b(WG);
 	switch (type) {
 	case XM_SCALAR_FLOAT:
@@ -363,6 +364,7 @@ launch_fill_random(const seg_desc *dsegs, size_t nseg, size_t nchunks,
 {
 	if (nchunks == 0)
 		return;
+	bump_data_epoch();
 	dim3 g((unsigned)nchunks), b(WG);
 	switch (type) {
 	case XM_SCALAR_FLOAT:
@@ -451,6 +453,7 @@ launch_elem_packed(const void *djobs, size_t njobs, size_t nchunks, int op,
 {
 	if (nchunks == 0 || njobs == 0)
 		return;
+	bump_data_epoch();
 	const elem_job_dev *dj = (const elem_job_dev *)djobs;
 	uint32_t nj = (uint32_t)njobs;
 #define PAIR(DT, ST, D, S) \
