@@ -295,8 +295,8 @@ def main():
                          "achieved": F * nprod / (sgemm * 1e-3) / 1e12,
                          "peak": BF16_PEAK, "unit": "TFLOP/s (BF16 MFMA)",
                          "frac": F * nprod / (sgemm * 1e-3) / 1e12 / BF16_PEAK,
-                         "traffic": measured_traffic("k_gemm_split"),
-                         "kernel": "k_gemm_split", "kernel_ms": sgemm}}
+                         "traffic": measured_traffic("k_gemm_split2"),
+                         "kernel": "k_gemm_split2", "kernel_ms": sgemm}}
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(

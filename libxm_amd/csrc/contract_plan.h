@@ -90,5 +90,9 @@ void split_prepare(contract_plan *pl, const xm_tensor_t *a,
 void split_slices(contract_plan *pl, hipStream_t s);
 void split_gemm(contract_plan *pl, hipStream_t s);
 void split_release(contract_plan *pl);
+/* Whether a split contraction with this many slices runs the LDS-staged
+ * kernel (128 x 128 tiles; slices 3-6) rather than the per-wave one (64 x 64
+ * tiles; slices 7-8, or any count with XM_SPLIT_KERNEL=1). */
+bool split_staged_kernel(int nslices);
 
 #endif /* XM_CONTRACT_PLAN_H */

@@ -288,11 +288,17 @@ k_gemm_simple(const gemm_tile *__restrict__ tiles,
 
 /* ---- tiles and launch --------------------------------------------- */
 
-static int
-tile_edge(const contract_plan *pl)
+/* Output tile (P x Q) of the kernel the plan will launch. */
+static void
+tile_shape(const contract_plan *pl, uint32_t *bp, uint32_t *bq)
 {
-	(void)pl;
-	return 64;
+	*bp = 64;
+	*bq = 64;
+	if (pl->precision == XM_PRECISION_SPLIT_BF16 &&
+	    split_staged_kernel(pl->nslices)) {
+		*bp = 128;
+		*bq = 128;
+	}
 }
 
 /*
@@ -308,15 +314,16 @@ void
 build_tiles(contract_plan *pl)
 {
 	const int NXCD = 8;
-	int B = tile_edge(pl);
-	pl->tile_cfg = B;
+	uint32_t BP, BQ;
+	tile_shape(pl, &BP, &BQ);
+	pl->tile_cfg = (int)(BP * 1000 + BQ);
 	std::vector<std::pair<double, uint32_t>> order;
 	for (uint32_t i = 0; i < pl->cblocks.size(); i++) {
 		const gemm_cblock &cb = pl->cblocks[i];
 		double kt = 1;
 		for (uint32_t p = cb.pair_begin; p < cb.pair_end; p++)
 			kt += pl->pairs[p].k;
-		double nt = (double)((cb.p + B - 1) / B) * ((cb.q + B - 1) / B);
+		double nt = (double)((cb.p + BP - 1) / BP) * ((cb.q + BQ - 1) / BQ);
 		order.push_back({kt * nt, i});
 	}
 	std::stable_sort(order.begin(), order.end(),
@@ -328,8 +335,8 @@ build_tiles(contract_plan *pl)
 		int x = (int)(std::min_element(load, load + NXCD) - load);
 		load[x] += o.first;
 		const gemm_cblock &cb = pl->cblocks[o.second];
-		for (uint32_t q0 = 0; q0 < cb.q; q0 += B)
-			for (uint32_t p0 = 0; p0 < cb.p; p0 += B)
+		for (uint32_t q0 = 0; q0 < cb.q; q0 += BQ)
+			for (uint32_t p0 = 0; p0 < cb.p; p0 += BP)
 				bins[x].push_back({o.second, p0, q0, 0});
 	}
 	size_t per = 0;

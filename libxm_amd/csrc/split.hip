@@ -45,15 +45,14 @@ struct split_operand {
 	uint64_t src;		/* FP64 panel */
 	uint32_t mode, ld, rows, k;
 	uint32_t rows_pad, kpad, nchunk, pad;
-	uint64_t slices;	/* nslices planes of rows_pad * kpad bf16 */
-	uint64_t exps;		/* rows_pad * nchunk int32 */
+	uint64_t slices;	/* nslices * rows_pad * kpad bf16, fragment order */
+	uint64_t exps;		/* int32 [nchunk][rows_pad] */
 	uint64_t first_wg;	/* prefix of (rows_pad/32) * nchunk */
 };
 
 struct split_pair {
-	uint64_t xs, ys;	/* slice plane 0 of X and Y */
+	uint64_t xs, ys;	/* slices of X and Y */
 	uint64_t xe, ye;	/* exponents of X and Y */
-	uint64_t xplane, yplane;	/* elements per slice plane */
 	uint32_t k, nchunk, xrows_pad, yrows_pad;
 	uint32_t nks, pad;	/* 16-wide k steps in kpad */
 	double a_re;
@@ -139,14 +138,13 @@ k_slice(const split_operand *__restrict__ ops, uint32_t nops)
 		(void)frexp(m, &ex);
 	const uint32_t grow = rt * 32 + r;
 	if (part == 0)
-		((int *)(uintptr_t)op.exps)[(size_t)grow * op.nchunk + c] = ex;
+		((int *)(uintptr_t)op.exps)[(size_t)c * op.rows_pad + grow] = ex;
 
 	/* this thread: row r, k in [c*KC + 16*part, +16) */
 	const uint32_t k0 = c * KC + 16 * part;
 	if (k0 >= op.kpad)
 		return;
 	const uint32_t nks = op.kpad / 16, ks = k0 / 16;
-	const size_t plane = (size_t)op.rows_pad * op.kpad;
 	uint16_t dig[S][16];
 #pragma unroll
 	for (int j = 0; j < 16; j++) {
@@ -162,12 +160,13 @@ k_slice(const split_operand *__restrict__ ops, uint32_t nops)
 			dig[t][j] = bf16_bits_exact(ldexp(d, -8 - 9 * t));
 		}
 	}
-	/* fragment order: ((rt * nks + ks) * 64 + lane) * 8 + j, lane = row +
-	 * 32 * (k % 16 >= 8) */
+	/* fragment order: (((rt * nks + ks) * S + t) * 64 + lane) * 8 + j,
+	 * lane = row + 32 * (k % 16 >= 8): the S slices of one row tile and
+	 * k step are one contiguous S KiB run */
 	uint16_t *base = (uint16_t *)(uintptr_t)op.slices;
-	const size_t f = ((size_t)rt * nks + ks) * 64;
 #pragma unroll
 	for (int t = 0; t < S; t++) {
+		const size_t f = (((size_t)rt * nks + ks) * S + t) * 64;
 		uint4 lo, hi;
 		lo.x = dig[t][0] | ((uint32_t)dig[t][1] << 16);
 		lo.y = dig[t][2] | ((uint32_t)dig[t][3] << 16);
@@ -177,9 +176,8 @@ k_slice(const split_operand *__restrict__ ops, uint32_t nops)
 		hi.y = dig[t][10] | ((uint32_t)dig[t][11] << 16);
 		hi.z = dig[t][12] | ((uint32_t)dig[t][13] << 16);
 		hi.w = dig[t][14] | ((uint32_t)dig[t][15] << 16);
-		uint16_t *p = base + t * plane;
-		*(uint4 *)(p + (f + r) * 8) = lo;
-		*(uint4 *)(p + (f + r + 32) * 8) = hi;
+		*(uint4 *)(base + (f + r) * 8) = lo;
+		*(uint4 *)(base + (f + r + 32) * 8) = hi;
 	}
 }
 
@@ -194,8 +192,279 @@ ld_frag(const uint16_t *p)
 	return r;
 }
 
+/*
+ * Workgroup tile 128 (P) x 128 (Q), 8 waves, one workgroup per CU (two
+ * waves per SIMD).  Wave w owns P rows (w & 3) * 32 + [0,32) and Q columns
+ * (w >> 2) * 64 + [0,64) as two 32 x 32 MFMA tiles.  For every 16-wide k
+ * step the workgroup stages the S slices of its 4 X and 4 Y row tiles (8S
+ * fragments of 1 KiB) and the exponents of the step's chunk in LDS.  A
+ * fragment is a contiguous 1 KiB run, so it is one wave-wide
+ * global_load_lds_dwordx4 (lane l's 16 bytes land at the LDS base + 16 l):
+ * no staging registers.  NST stages rotate with the DMA running NST - 1
+ * steps ahead; every wave issues exactly S + 1 DMAs per step (row tiles
+ * past an operand's padding read a zero page), so the end of a step waits
+ * with a constant count for the next step's data and leaves the later
+ * steps' loads in flight; one barrier per step.  The (pair, k step)
+ * sequence of the C block is walked flat, so the prefetch crosses pair
+ * boundaries.  At each 128-k chunk end the weight classes are recombined
+ * into FP64 register sums; C is read and written once, at the end.
+ */
+#define SP_BP 128
+#define SP_BQ 128
+#define SP_WG 512
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void gbl_void_t;
+
+/* source of the DMAs of row tiles past an operand's padding */
+static __device__ uint4 sp_zero_page[64];
+
+__device__ __forceinline__ bf8
+frag_of(uint4 v)
+{
+	bf8 r;
+	__builtin_memcpy(&r, &v, 16);
+	return r;
+}
+
+/* One LDS stage: the 8S fragments of a k step, then the exponents of the
+ * step's chunk for the tile's 128 X rows and 128 Y rows. */
+template <int S>
+struct sp_stage {
+	uint4 frag[8 * S * 64];
+	int ex[SP_BP];
+	int ey[SP_BQ];
+};
+
+/* stages that fit the 160 KiB of LDS */
+template <int S>
+constexpr int
+sp_nstages(void)
+{
+	return sizeof(sp_stage<S>) * 3 <= 160 * 1024 ? 3 : 2;
+}
+
+template <int S>
+__global__ void __launch_bounds__(SP_WG)
+k_gemm_split2(const gemm_tile *__restrict__ tiles,
+    const gemm_cblock *__restrict__ cbs, const split_pair *__restrict__ pairs,
+    const gemm_scalars *__restrict__ scp)
+{
+	constexpr int FX = 4 * S, NST = sp_nstages<S>();
+	/* s_waitcnt: vmcnt = the DMAs of the NST - 2 later steps, expcnt and
+	 * lgkmcnt not waited on */
+	constexpr int WAIT = ((NST - 2) * (S + 1)) | (0x7 << 4) | (0xf << 8);
+	__shared__ sp_stage<S> st[NST];
+
+	const gemm_tile tl = tiles[blockIdx.x];
+	if (tl.cblock == 0xffffffffu)
+		return;
+	const gemm_cblock cb = cbs[tl.cblock];
+	const gemm_scalars sc = *scp;
+	const int tid = threadIdx.x, lane = tid & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+	const int wp = wave & 3, wq = wave >> 2;
+	const uint32_t pw = tl.p0 + wp * 32, qw = tl.q0 + wq * 64;
+	const uint32_t pend = cb.pair_end;
+	const uint32_t pbeg = (sc.flags & 1) ? pend : cb.pair_begin;
+	const uint32_t prow = pw + (lane & 31);
+	const uint32_t rx0 = tl.p0 / 32, ry0 = tl.q0 / 32;
+
+	f16v a2[2] = {}, a3[2] = {}, a4[2] = {}, a5[2] = {};
+	double acc[2][16];
+#pragma unroll
+	for (int j = 0; j < 2; j++)
+#pragma unroll
+		for (int i = 0; i < 16; i++)
+			acc[j][i] = 0.0;
+
+	/*
+	 * k step kx of pair px into stage nx: S + 1 DMAs per wave.  Fragment
+	 * f = wave + 8 i is wave's i-th (1 KiB); a row tile past the operand's
+	 * padded rows reads the zero page.  Then exponent quarter wave % 4 (64
+	 * rows: X rows 0-63, 64-127, Y rows 0-63, 64-127 of the tile) of the
+	 * step's chunk, 4 bytes a lane; waves 4-7 repeat waves 0-3's, writing
+	 * the same values.  Rows past the padding read row 0 and are never used.
+	 */
+	auto fetch = [&](sp_stage<S> *__restrict__ nx, const split_pair px,
+	    uint32_t kx) {
+#pragma unroll
+		for (int i = 0; i < S; i++) {
+			const int f = wave + 8 * i;
+			const bool isx = f < FX;
+			const int g = isx ? f : f - FX;
+			const uint32_t rt = (isx ? rx0 : ry0) + g / S;
+			const int t = g % S;
+			const uint4 *src = &sp_zero_page[lane];
+			if (rt * 32 < (isx ? px.xrows_pad : px.yrows_pad))
+				src = (const uint4 *)(uintptr_t)(isx ? px.xs : px.ys) +
+				    (((size_t)rt * px.nks + kx) * S + t) * 64 + lane;
+			__builtin_amdgcn_global_load_lds((gbl_void_t *)src,
+			    (lds_void_t *)&nx->frag[f * 64], 16, 0, 0);
+		}
+		const int quarter = wave & 3;
+		const bool isx = quarter < 2;
+		const uint32_t rows = isx ? px.xrows_pad : px.yrows_pad;
+		uint32_t row = (isx ? tl.p0 : tl.q0) + 64 * (quarter & 1) + lane;
+		if (row >= rows)
+			row = 0;
+		const int *src = (const int *)(uintptr_t)(isx ? px.xe : px.ye) +
+		    (size_t)(kx / (KC / 16)) * rows + row;
+		int *dst = (isx ? nx->ex : nx->ey) + 64 * (quarter & 1);
+		__builtin_amdgcn_global_load_lds((gbl_void_t *)src,
+		    (lds_void_t *)dst, 4, 0, 0);
+	};
+	/* chunk done: acc += al * 2^(ex + ey) * (((a5 + a4) + a3) + a2); a
+	 * lane's 4 consecutive q rows are one int4 of the stage's exponents */
+	auto flush = [&](const sp_stage<S> *__restrict__ cur,
+	    const split_pair px) {
+		const double al = __dmul_rn(sc.alpha_re, px.a_re);
+		const int ep = prow < px.xrows_pad ?
+		    cur->ex[wp * 32 + (lane & 31)] : 0;
+#pragma unroll
+		for (int j = 0; j < 2; j++)
+#pragma unroll
+			for (int g = 0; g < 4; g++) {
+				const uint32_t ql = wq * 64 + 32 * j + 8 * g +
+				    4 * (lane >> 5);
+				const int4 e4 = tl.q0 + ql < px.yrows_pad ?
+				    *(const int4 *)(cur->ey + ql) :
+				    make_int4(0, 0, 0, 0);
+				const int eq[4] = {e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+				for (int r = 0; r < 4; r++) {
+					const int i = 4 * g + r;
+					const double v = (((double)a5[j][i] +
+					    (double)a4[j][i]) + (double)a3[j][i]) +
+					    (double)a2[j][i];
+					acc[j][i] = fma(ldexp(al, ep + eq[r]), v,
+					    acc[j][i]);
+				}
+			}
+#pragma unroll
+		for (int j = 0; j < 2; j++) {
+			a2[j] = f16v{};
+			a3[j] = f16v{};
+			a4[j] = f16v{};
+			a5[j] = f16v{};
+		}
+	};
+
+	/*
+	 * Descriptors are values, never references into two address spaces,
+	 * so they stay in scalar registers and the loop stays uniform.  The
+	 * stages reach step() as restrict pointers: the alias scopes that
+	 * inlining gives them let the compiler read this stage while the DMA
+	 * into the other is in flight (without them it waits for the DMA
+	 * before the first read, serialising every step).
+	 */
+	auto advance = [&](uint32_t &i, uint32_t &k, split_pair &p) {
+		if (++k >= p.nks) {
+			k = 0;
+			if (++i < pend)
+				p = pairs[i];
+		}
+	};
+	/* compute cursor (ip, ks, pr); fetch cursor NST - 1 steps ahead */
+	uint32_t ip = pbeg, ks = 0, fip = pbeg, fks = 0;
+	split_pair pr, fpr;
+	if (ip < pend) {
+		pr = pairs[ip];
+		fpr = pr;
+	}
+#pragma unroll
+	for (int s = 0; s < NST - 1; s++)
+		if (fip < pend) {
+			fetch(&st[s], fpr, fks);
+			advance(fip, fks, fpr);
+		}
+	__builtin_amdgcn_s_waitcnt(0);
+	__syncthreads();
+	auto step = [&](const sp_stage<S> *__restrict__ cur,
+	    sp_stage<S> *__restrict__ nxt) {
+		const bool more = fip < pend;
+		if (more) {
+			fetch(nxt, fpr, fks);
+			advance(fip, fks, fpr);
+		}
+		/* MFMAs of this k step: D[q][p] += Y_u * X_t by weight class */
+		bf8 fx[S];
+#pragma unroll
+		for (int t = 0; t < S; t++)
+			fx[t] = frag_of(cur->frag[(wp * S + t) * 64 + lane]);
+#pragma unroll
+		for (int u = 1; u <= S; u++) {
+			const bf8 fy0 = frag_of(cur->frag[(FX + (2 * wq) * S + u - 1) * 64 + lane]);
+			const bf8 fy1 = frag_of(cur->frag[(FX + (2 * wq + 1) * S + u - 1) * 64 + lane]);
+#pragma unroll
+			for (int t = 1; t + u <= S + 1; t++) {
+				const int d = t + u;
+				if (d == 2) {
+					a2[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fy0, fx[t - 1], a2[0], 0, 0, 0);
+					a2[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fy1, fx[t - 1], a2[1], 0, 0, 0);
+				} else if (d == 3) {
+					a3[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fy0, fx[t - 1], a3[0], 0, 0, 0);
+					a3[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fy1, fx[t - 1], a3[1], 0, 0, 0);
+				} else if (d == 4) {
+					a4[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fy0, fx[t - 1], a4[0], 0, 0, 0);
+					a4[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fy1, fx[t - 1], a4[1], 0, 0, 0);
+				} else {
+					a5[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fy0, fx[t - 1], a5[0], 0, 0, 0);
+					a5[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fy1, fx[t - 1], a5[1], 0, 0, 0);
+				}
+			}
+		}
+		if ((ks % (KC / 16)) == KC / 16 - 1 || ks + 1 == pr.nks)
+			flush(cur, pr);
+		/*
+		 * This step's stage is read (every LDS read's value is used by
+		 * now) and this wave's DMAs of the next step have landed; those
+		 * just issued stay in flight.  A bare s_barrier then publishes
+		 * the stage: __syncthreads()' release fence would also wait for
+		 * the DMAs in flight.  The signal fences only stop the compiler
+		 * from moving LDS accesses across the barrier.
+		 */
+		if (more)
+			__builtin_amdgcn_s_waitcnt(WAIT);
+		else
+			__builtin_amdgcn_s_waitcnt(0);
+		__atomic_signal_fence(__ATOMIC_SEQ_CST);
+		__builtin_amdgcn_s_barrier();
+		__atomic_signal_fence(__ATOMIC_SEQ_CST);
+		advance(ip, ks, pr);
+	};
+	int cs = 0;
+	while (ip < pend) {
+		const int ns = cs + NST - 1 < NST ? cs + NST - 1 : cs - 1;
+		step(&st[cs], &st[ns]);
+		cs = cs + 1 < NST ? cs + 1 : 0;
+	}
+	double *C = (double *)(uintptr_t)cb.c;
+	const bool bz = sc.flags & 2, none = pbeg == pend;
+#pragma unroll
+	for (int j = 0; j < 2; j++)
+#pragma unroll
+		for (int i = 0; i < 16; i++) {
+			const uint32_t q = qw + 32 * j + 8 * (i / 4) +
+			    4 * (lane >> 5) + (i % 4);
+			if (prow < cb.p && q < cb.q) {
+				const size_t o = (size_t)prow + (size_t)q * cb.ldc;
+				double v;
+				if (bz)
+					v = none ? 0.0 : acc[j][i];
+				else if (none)
+					v = __dmul_rn(sc.beta_re, C[o]);
+				else
+					v = __dadd_rn(__dmul_rn(sc.beta_re, C[o]),
+					    acc[j][i]);
+				C[o] = v;
+			}
+		}
+}
+
 /* 64 x 64 outputs per workgroup; wave w computes the 32 x 32 block
- * (p: (w & 1) * 32, q: (w >> 1) * 32) with one 32x32 MFMA tile. */
+ * (p: (w & 1) * 32, q: (w >> 1) * 32) with one 32x32 MFMA tile.  Kept as
+ * the reference form of k_gemm_split2 (XM_SPLIT_KERNEL=1). */
 template <int S>
 __global__ void __launch_bounds__(SPLIT_WG)
 k_gemm_split(const gemm_tile *__restrict__ tiles,
@@ -223,8 +492,8 @@ k_gemm_split(const gemm_tile *__restrict__ tiles,
 			const uint16_t *ys = (const uint16_t *)(uintptr_t)pr.ys;
 			const int *xe = (const int *)(uintptr_t)pr.xe;
 			const int *ye = (const int *)(uintptr_t)pr.ye;
-			const size_t xf = (size_t)(pw / 32) * pr.nks * 64 + lane;
-			const size_t yf = (size_t)(qw / 32) * pr.nks * 64 + lane;
+			const size_t xf = (size_t)(pw / 32) * pr.nks;
+			const size_t yf = (size_t)(qw / 32) * pr.nks;
 			for (uint32_t c = 0; c < pr.nchunk; c++) {
 				/* one FP32 accumulator per weight class d = t + u:
 				 * d = 2, 3, 4 and d >= 5 */
@@ -235,10 +504,10 @@ k_gemm_split(const gemm_tile *__restrict__ tiles,
 					bf8 fx[S], fy[S];
 #pragma unroll
 					for (int t = 0; t < S; t++) {
-						fx[t] = ld_frag(xs + t * pr.xplane +
-						    (xf + (size_t)ks * 64) * 8);
-						fy[t] = ld_frag(ys + t * pr.yplane +
-						    (yf + (size_t)ks * 64) * 8);
+						fx[t] = ld_frag(xs + (((xf + ks) * S + t) *
+						    64 + lane) * 8);
+						fy[t] = ld_frag(ys + (((yf + ks) * S + t) *
+						    64 + lane) * 8);
 					}
 					/* D[i = q][j = p]: A = Y slice, B = X slice */
 #pragma unroll
@@ -264,12 +533,12 @@ k_gemm_split(const gemm_tile *__restrict__ tiles,
 				}
 				/* recombine, smallest weights first: lane -> p row,
 				 * register -> q row */
-				const int exp_p = xe[(size_t)(pw + (lane & 31)) *
-				    pr.nchunk + c];
+				const int exp_p = xe[(size_t)c * pr.xrows_pad +
+				    pw + (lane & 31)];
 #pragma unroll
 				for (int i = 0; i < 16; i++) {
 					uint32_t qq = qw + 8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
-					int exp_q = ye[(size_t)qq * pr.nchunk + c];
+					int exp_q = ye[(size_t)c * pr.yrows_pad + qq];
 					double v = (((double)a5[i] + (double)a4[i]) +
 					    (double)a3[i]) + (double)a2[i];
 					acc[i] = fma(ldexp(al, exp_p + exp_q), v, acc[i]);
@@ -370,8 +639,6 @@ split_prepare(contract_plan *pl, const xm_tensor_t *, const xm_tensor_t *,
 		sp.ys = y.slices;
 		sp.xe = x.exps;
 		sp.ye = y.exps;
-		sp.xplane = (uint64_t)x.rows_pad * x.kpad;
-		sp.yplane = (uint64_t)y.rows_pad * y.kpad;
 		sp.k = x.k;
 		sp.nchunk = x.nchunk;
 		sp.xrows_pad = x.rows_pad;
@@ -395,15 +662,30 @@ split_prepare(contract_plan *pl, const xm_tensor_t *, const xm_tensor_t *,
 	pl->unfold_bytes += 0;
 }
 
-#define SDISPATCH(KERNEL, S, ...)					\
+/* KERNEL<S><<<launch>>>(args) for the runtime slice count S. */
+#define SDISPATCH(KERNEL, S, ...) SDISPATCH_ARGS(KERNEL, S, (__VA_ARGS__))
+#define SDISPATCH_ARGS(KERNEL, S, LAUNCH) [&](auto... args) {		\
 	switch (S) {							\
-	case 3: KERNEL<3><<<__VA_ARGS__>>>; break;			\
-	case 4: KERNEL<4><<<__VA_ARGS__>>>; break;			\
-	case 5: KERNEL<5><<<__VA_ARGS__>>>; break;			\
-	case 6: KERNEL<6><<<__VA_ARGS__>>>; break;			\
-	case 7: KERNEL<7><<<__VA_ARGS__>>>; break;			\
-	default: KERNEL<8><<<__VA_ARGS__>>>; break;			\
+	case 3: KERNEL<3><<<SD_UNPACK LAUNCH>>>(args...); break;	\
+	case 4: KERNEL<4><<<SD_UNPACK LAUNCH>>>(args...); break;	\
+	case 5: KERNEL<5><<<SD_UNPACK LAUNCH>>>(args...); break;	\
+	case 6: KERNEL<6><<<SD_UNPACK LAUNCH>>>(args...); break;	\
+	case 7: KERNEL<7><<<SD_UNPACK LAUNCH>>>(args...); break;	\
+	default: KERNEL<8><<<SD_UNPACK LAUNCH>>>(args...); break;	\
+	}								\
+}
+#define SD_UNPACK(...) __VA_ARGS__
+
+bool
+split_staged_kernel(int nslices)
+{
+	static int reference = -1;
+	if (reference < 0) {
+		const char *e = getenv("XM_SPLIT_KERNEL");
+		reference = e && *e == '1';
 	}
+	return !reference && nslices >= 3 && nslices <= 6;
+}
 
 void
 split_slices(contract_plan *pl, hipStream_t s)
@@ -434,14 +716,21 @@ split_gemm(contract_plan *pl, hipStream_t s)
 	pl->gemm_launches = 0;
 	if (ss == nullptr || nt == 0)
 		return;
-	dim3 g((unsigned)nt), b(SPLIT_WG);
-	switch (ss->nslices) {
-	case 3: k_gemm_split<3><<<g, b, 0, s>>>(pl->d_tiles, pl->d_cblocks, ss->d_pairs, pl->d_scalars); break;
-	case 4: k_gemm_split<4><<<g, b, 0, s>>>(pl->d_tiles, pl->d_cblocks, ss->d_pairs, pl->d_scalars); break;
-	case 5: k_gemm_split<5><<<g, b, 0, s>>>(pl->d_tiles, pl->d_cblocks, ss->d_pairs, pl->d_scalars); break;
-	case 6: k_gemm_split<6><<<g, b, 0, s>>>(pl->d_tiles, pl->d_cblocks, ss->d_pairs, pl->d_scalars); break;
-	case 7: k_gemm_split<7><<<g, b, 0, s>>>(pl->d_tiles, pl->d_cblocks, ss->d_pairs, pl->d_scalars); break;
-	default: k_gemm_split<8><<<g, b, 0, s>>>(pl->d_tiles, pl->d_cblocks, ss->d_pairs, pl->d_scalars); break;
+	const dim3 g((unsigned)nt);
+	const gemm_tile *t = pl->d_tiles;
+	const gemm_cblock *c = pl->d_cblocks;
+	const gemm_scalars *sc = pl->d_scalars;
+	if (!split_staged_kernel(ss->nslices)) {
+		SDISPATCH(k_gemm_split, ss->nslices, g, dim3(SPLIT_WG), 0, s)(t,
+		    c, ss->d_pairs, sc);
+	} else {
+		const dim3 b(SP_WG);
+		switch (ss->nslices) {
+		case 3: k_gemm_split2<3><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
+		case 4: k_gemm_split2<4><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
+		case 5: k_gemm_split2<5><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
+		default: k_gemm_split2<6><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
+		}
 	}
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess)
