@@ -23,27 +23,38 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 #define GEMM_WG 256
 
+typedef const __attribute__((address_space(1))) double gdouble;
+
+/* Element (p, k) of a panel if inside P x K, else element 0 (to be
+ * discarded by the caller).  A global (not flat) load, so it counts only
+ * against vmcnt and LDS waits do not wait for it; no branch. */
 __device__ __forceinline__ double
-ld_x(const double *x, uint8_t mode, uint32_t ld, uint32_t p, uint32_t k,
+ld_raw(const double *x, uint32_t mode, uint32_t ld, uint32_t p, uint32_t k,
     uint32_t P, uint32_t K)
 {
-	if (p >= P || k >= K)
-		return 0.0;
-	return mode == PANEL_PCONTIG ? x[(size_t)p + (size_t)k * ld] :
-	    x[(size_t)k + (size_t)p * ld];
+	const bool in = p < P && k < K;
+	const size_t off = mode == PANEL_PCONTIG ?
+	    (size_t)p + (size_t)k * ld : (size_t)k + (size_t)p * ld;
+	return ((gdouble *)x)[in ? off : 0];
 }
 
 /*
  * FP64 tile kernel.  BP x BQ outputs per workgroup, 4 waves as 2 x 2, each
- * wave (BP/2) x (BQ/2) = MT x NT MFMA tiles of 16 x 16.  BK = 16.
+ * wave (BP/2) x (BQ/2) = MT x NT MFMA tiles of 16 x 16, BK k per LDS step.
+ * One LDS buffer and a small register footprint keep several waves per
+ * SIMD resident (3 at 64 x 128 x 16): with 64-cycle MFMAs it is the other
+ * waves that hide a wave's loads, stores and barriers.  Measured on the
+ * o=64 v=384 ladder, 64 x 128 x 16 beats 64 x 64 x {8,16,32}, 128 x 64,
+ * 64 x 192, 64 x 256, 64 x 128 x {8,32} and 128 x 128 tiles with a
+ * register-prefetched double buffer (2 waves per SIMD).  A wave whose rows
+ * or columns all lie past the C block skips its MFMAs.
  */
-template <int BP, int BQ>
+template <int BP, int BQ, int BK>
 __global__ void __launch_bounds__(GEMM_WG)
 k_gemm_f64(const gemm_tile *__restrict__ tiles,
     const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
     const gemm_scalars *__restrict__ scp)
 {
-	constexpr int BK = 16;
 	constexpr int MT = BP / 2 / 16, NT = BQ / 2 / 16;
 	constexpr int EX = BP * BK / GEMM_WG, EY = BQ * BK / GEMM_WG;
 	constexpr int LP = BP + 1, LQ = BQ + 1;
@@ -55,66 +66,83 @@ k_gemm_f64(const gemm_tile *__restrict__ tiles,
 		return;
 	const gemm_cblock cb = cbs[tl.cblock];
 	const gemm_scalars sc = *scp;
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int tid = threadIdx.x, lane = tid & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const int wp = (wave & 1) * (BP / 2), wq = (wave >> 1) * (BQ / 2);
+	const bool live = tl.p0 + wp < cb.p && tl.q0 + wq < cb.q;
 
 	d4 acc[MT][NT];
 	for (int i = 0; i < MT; i++)
 		for (int j = 0; j < NT; j++)
 			acc[i][j] = d4{0, 0, 0, 0};
 
+	/* step k0 of pair pr: load() fetches the raw values into registers
+	 * rx / ry; store() zeroes those outside the panels, scales X by al
+	 * and writes the LDS tiles. */
+	double rx[EX], ry[EY];
+	auto load = [&](const gemm_pair pr, uint32_t k0) {
+		const double *X = (const double *)(uintptr_t)pr.x;
+		const double *Y = (const double *)(uintptr_t)pr.y;
+#pragma unroll
+		for (int e = 0; e < EX; e++) {
+			int idx = tid + GEMM_WG * e, p, k;
+			if (pr.xmode == PANEL_PCONTIG) {
+				p = idx % BP; k = idx / BP;
+			} else {
+				k = idx % BK; p = idx / BK;
+			}
+			rx[e] = ld_raw(X, pr.xmode, pr.ldx, tl.p0 + p, k0 + k,
+			    cb.p, pr.k);
+		}
+#pragma unroll
+		for (int e = 0; e < EY; e++) {
+			int idx = tid + GEMM_WG * e, q, k;
+			if (pr.ymode == PANEL_PCONTIG) {
+				q = idx % BQ; k = idx / BQ;
+			} else {
+				k = idx % BK; q = idx / BK;
+			}
+			ry[e] = ld_raw(Y, pr.ymode, pr.ldy, tl.q0 + q, k0 + k,
+			    cb.q, pr.k);
+		}
+	};
+	auto store = [&](const gemm_pair pr, uint32_t k0) {
+		const double al = __dmul_rn(sc.alpha_re, pr.a_re);
+#pragma unroll
+		for (int e = 0; e < EX; e++) {
+			int idx = tid + GEMM_WG * e, p, k;
+			if (pr.xmode == PANEL_PCONTIG) {
+				p = idx % BP; k = idx / BP;
+			} else {
+				k = idx % BK; p = idx / BK;
+			}
+			const bool in = tl.p0 + p < cb.p && k0 + k < pr.k;
+			Xs[k * LP + p] = in ? rx[e] * al : 0.0;
+		}
+#pragma unroll
+		for (int e = 0; e < EY; e++) {
+			int idx = tid + GEMM_WG * e, q, k;
+			if (pr.ymode == PANEL_PCONTIG) {
+				q = idx % BQ; k = idx / BQ;
+			} else {
+				k = idx % BK; q = idx / BK;
+			}
+			const bool in = tl.q0 + q < cb.q && k0 + k < pr.k;
+			Ys[k * LQ + q] = in ? ry[e] : 0.0;
+		}
+	};
+
+	/* descriptors are values (scalar registers), never references */
 	const uint32_t pbeg = (sc.flags & 1) ? cb.pair_end : cb.pair_begin;
 	for (uint32_t ip = pbeg; ip < cb.pair_end; ip++) {
 		const gemm_pair pr = pairs[ip];
-		const double al = __dmul_rn(sc.alpha_re, pr.a_re);
-		const double *X = (const double *)(uintptr_t)pr.x;
-		const double *Y = (const double *)(uintptr_t)pr.y;
 		for (uint32_t k0 = 0; k0 < pr.k; k0 += BK) {
-			double rx[EX], ry[EY];
-#pragma unroll
-			for (int e = 0; e < EX; e++) {
-				int idx = tid + GEMM_WG * e, p, k;
-				if (pr.xmode == PANEL_PCONTIG) {
-					p = idx % BP; k = idx / BP;
-				} else {
-					k = idx % BK; p = idx / BK;
-				}
-				rx[e] = ld_x(X, pr.xmode, pr.ldx, tl.p0 + p, k0 + k,
-				    cb.p, pr.k) * al;
-			}
-#pragma unroll
-			for (int e = 0; e < EY; e++) {
-				int idx = tid + GEMM_WG * e, q, k;
-				if (pr.ymode == PANEL_PCONTIG) {
-					q = idx % BQ; k = idx / BQ;
-				} else {
-					k = idx % BK; q = idx / BK;
-				}
-				ry[e] = ld_x(Y, pr.ymode, pr.ldy, tl.q0 + q, k0 + k,
-				    cb.q, pr.k);
-			}
+			load(pr, k0);
 			__syncthreads();
-#pragma unroll
-			for (int e = 0; e < EX; e++) {
-				int idx = tid + GEMM_WG * e, p, k;
-				if (pr.xmode == PANEL_PCONTIG) {
-					p = idx % BP; k = idx / BP;
-				} else {
-					k = idx % BK; p = idx / BK;
-				}
-				Xs[k * LP + p] = rx[e];
-			}
-#pragma unroll
-			for (int e = 0; e < EY; e++) {
-				int idx = tid + GEMM_WG * e, q, k;
-				if (pr.ymode == PANEL_PCONTIG) {
-					q = idx % BQ; k = idx / BQ;
-				} else {
-					k = idx % BK; q = idx / BK;
-				}
-				Ys[k * LQ + q] = ry[e];
-			}
+			store(pr, k0);
 			__syncthreads();
+			if (!live)
+				continue;
 #pragma unroll
 			for (int kk = 0; kk < BK; kk += 4) {
 				const int kr = kk + (lane >> 4), c16 = lane & 15;
@@ -288,15 +316,29 @@ k_gemm_simple(const gemm_tile *__restrict__ tiles,
 
 /* ---- tiles and launch --------------------------------------------- */
 
+static bool
+use_simple(void)
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("XM_GEMM_SIMPLE");
+		v = e && *e == '1';
+	}
+	return v;
+}
+
 /* Output tile (P x Q) of the kernel the plan will launch. */
 static void
 tile_shape(const contract_plan *pl, uint32_t *bp, uint32_t *bq)
 {
 	*bp = 64;
 	*bq = 64;
-	if (pl->precision == XM_PRECISION_SPLIT_BF16 &&
-	    split_staged_kernel(pl->nslices)) {
-		*bp = 128;
+	if (pl->precision == XM_PRECISION_SPLIT_BF16) {
+		if (split_staged_kernel(pl->nslices)) {
+			*bp = 128;
+			*bq = 128;
+		}
+	} else if (pl->type == XM_SCALAR_DOUBLE && !use_simple()) {
 		*bq = 128;
 	}
 }
@@ -348,17 +390,6 @@ build_tiles(contract_plan *pl)
 			pl->tiles[s * NXCD + x] = bins[x][s];
 }
 
-static bool
-use_simple(void)
-{
-	static int v = -1;
-	if (v < 0) {
-		const char *e = getenv("XM_GEMM_SIMPLE");
-		v = e && *e == '1';
-	}
-	return v;
-}
-
 void
 launch_gemm(contract_plan *pl, hipStream_t s)
 {
@@ -373,7 +404,7 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 			k_gemm_simple<double><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		else
-			k_gemm_f64<64, 64><<<g, b, 0, s>>>(pl->d_tiles,
+			k_gemm_f64<64, 128, 16><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_FLOAT:

@@ -173,9 +173,9 @@ enum { PANEL_PCONTIG = 0, PANEL_KCONTIG = 1 };
 struct gemm_pair {
 	uint64_t x, y;		/* X panel (P x K), Y panel (Q x K) */
 	uint32_t k;		/* contracted extent */
-	uint8_t xmode, ymode;	/* PANEL_* */
-	uint16_t pad;
-	uint32_t ldx, ldy;
+	uint32_t xmode, ymode;	/* PANEL_*; whole dwords, so kernels read
+				 * them with scalar loads */
+	uint32_t ldx, ldy, pad;
 	double a_re, a_im;	/* merged pair factor s_a*s_b */
 };
 
