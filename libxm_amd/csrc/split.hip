@@ -39,6 +39,7 @@
 #define MAXS 8
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef const __attribute__((address_space(1))) double gdouble_t;
 typedef float f16v __attribute__((ext_vector_type(16)));
 
 struct split_operand {
@@ -79,12 +80,19 @@ bf16_bits_exact(double v)
 	return (uint16_t)(__float_as_uint(f) >> 16);
 }
 
+/*
+ * One workgroup per (32-row tile, 128-k chunk) of an operand panel; thread
+ * (r, part) = (tid % 32, tid / 32) owns row r and k in [16 part, +16) of the
+ * chunk and loads them straight from the panel: for a P-contiguous panel
+ * the 32 lanes of a row group read 256 consecutive bytes per k, for a
+ * K-contiguous one a lane reads a contiguous 128-byte run.  Only the row
+ * maximum goes through LDS.
+ */
 template <int S>
 __global__ void __launch_bounds__(SPLIT_WG)
 k_slice(const split_operand *__restrict__ ops, uint32_t nops)
 {
-	__shared__ double tile[32][KC + 1];
-	__shared__ double rmax[32][8];
+	__shared__ double rmax[8][32];
 	__shared__ uint32_t found;
 	const uint64_t wg = blockIdx.x;
 	if (threadIdx.x == 0) {
@@ -103,52 +111,54 @@ k_slice(const split_operand *__restrict__ ops, uint32_t nops)
 	const uint64_t local = wg - op.first_wg;
 	const uint32_t rt = (uint32_t)(local / op.nchunk);
 	const uint32_t c = (uint32_t)(local % op.nchunk);
-	const double *X = (const double *)(uintptr_t)op.src;
-	const int tid = threadIdx.x;
+	const gdouble_t *X = (const gdouble_t *)(uintptr_t)op.src;
+	const int tid = threadIdx.x, r = tid % 32, part = tid / 32;
+	const uint32_t grow = rt * 32 + r, k0 = c * KC + 16 * part;
 
-	/* load 32 rows x KC columns (zero outside the panel) */
-	for (int e = tid; e < 32 * KC; e += SPLIT_WG) {
-		int r, k;
-		if (op.mode == PANEL_PCONTIG) {
-			r = e % 32; k = e / 32;
-		} else {
-			k = e % KC; r = e / KC;
+	/* this thread's 16 values (zero outside the panel) */
+	double x[16];
+	const bool rin = grow < op.rows;
+	if (op.mode == PANEL_PCONTIG) {
+#pragma unroll
+		for (int j = 0; j < 16; j++) {
+			const bool in = rin && k0 + j < op.k;
+			const double v = X[in ? (size_t)grow +
+			    (size_t)(k0 + j) * op.ld : 0];
+			x[j] = in ? v : 0.0;
 		}
-		uint32_t gr = rt * 32 + r, gk = c * KC + k;
-		double v = 0.0;
-		if (gr < op.rows && gk < op.k)
-			v = op.mode == PANEL_PCONTIG ?
-			    X[(size_t)gr + (size_t)gk * op.ld] :
-			    X[(size_t)gk + (size_t)gr * op.ld];
-		tile[r][k] = v;
+	} else {
+		const gdouble_t *row = X + (size_t)grow * op.ld;
+#pragma unroll
+		for (int j = 0; j < 16; j++) {
+			const bool in = rin && k0 + j < op.k;
+			const double v = (in ? row : X)[in ? k0 + j : 0];
+			x[j] = in ? v : 0.0;
+		}
 	}
-	__syncthreads();
-	/* row maxima: 8 threads per row */
-	const int r = tid / 8, part = tid % 8;
+	/* row maxima over the chunk: 8 parts per row */
 	double m = 0.0;
-	for (int k = part * 16; k < part * 16 + 16; k++)
-		m = fmax(m, fabs(tile[r][k]));
-	rmax[r][part] = m;
+#pragma unroll
+	for (int j = 0; j < 16; j++)
+		m = fmax(m, fabs(x[j]));
+	rmax[part][r] = m;
 	__syncthreads();
 	m = 0.0;
+#pragma unroll
 	for (int q = 0; q < 8; q++)
-		m = fmax(m, rmax[r][q]);
+		m = fmax(m, rmax[q][r]);
 	int ex = 0;
 	if (m > 0.0)
 		(void)frexp(m, &ex);
-	const uint32_t grow = rt * 32 + r;
 	if (part == 0)
 		((int *)(uintptr_t)op.exps)[(size_t)c * op.rows_pad + grow] = ex;
 
-	/* this thread: row r, k in [c*KC + 16*part, +16) */
-	const uint32_t k0 = c * KC + 16 * part;
 	if (k0 >= op.kpad)
 		return;
 	const uint32_t nks = op.kpad / 16, ks = k0 / 16;
 	uint16_t dig[S][16];
 #pragma unroll
 	for (int j = 0; j < 16; j++) {
-		double y = ldexp(tile[r][16 * part + j], 8 - ex);
+		double y = ldexp(x[j], 8 - ex);
 		double d = rint(y);
 		dig[0][j] = bf16_bits_exact(ldexp(d, -8));
 		double rem = y - d;
