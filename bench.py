@@ -203,6 +203,130 @@ def copy_calibration(lib, steps):
                       "GBps": 2 * (8 << 29) / dt / 1e9}), flush=True)
 
 
+def canonical_bytes(spec, esize):
+    return sum(spec.block_size(i) for i in spec.canonical_blocks()) * esize
+
+
+def timed(lib, fn, steps):
+    """Median wall time (ms) of a synchronous library call."""
+    fn()
+    lib.synchronize()
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        fn()
+        lib.synchronize()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    return sorted(ts)[len(ts) // 2]
+
+
+def ring_mode(lib, args):
+    """Ring c(i,j,a,b) = sum t(i,k,a,c) w(k,b,c,j), o=96 v=480, block sizes
+    U[4,28] (~2.5e5 block pairs, unfolds of both operands), FP64 and
+    complex64; then xm_add(1, c, 0.5, c2) and xm_set(c, 0) on the result."""
+    from libxm_amd import xm, workloads
+    out = {"mode": "ring", "unit": "TFLOP/s", "forms": {}}
+    for name, tp, esize, mult in (("f64", xm.SCALAR_DOUBLE, 8, 2.0),
+                                  ("c64", xm.SCALAR_FLOAT_COMPLEX, 8, 8.0)):
+        w = workloads.ring_d2h(96, 480, args.seed, tp)
+        alloc = lib.allocator()
+        t, v, c = (s.build(lib, alloc) for s in w.specs)
+        for k, x in enumerate((t, v, c)):
+            lib.fill_random(x, args.seed * 16 + k)
+        tp0 = time.perf_counter()
+        lib.contract(w.alpha, t, v, w.beta, c, *w.idx)
+        lib.synchronize()
+        first = (time.perf_counter() - tp0) * 1e3
+        # two more calls finish the tile-shape tuning
+        ms, gemm_ms, prep_ms, st = time_steps(lib, w, t, v, c, args.steps,
+                                              max(args.warmup, 2))
+        F = st["flops"]         # 8 real flops per complex multiply-add
+        unf = st["panel_bytes"] + (st["cin_bytes"] if w.beta != 0 else 0.0)
+        f = {"ms_per_call": ms, "tflops": F / ms / 1e9, "flops": F,
+             "gemm_ms": gemm_ms, "gemm_tflops": F / gemm_ms / 1e9,
+             "block_pairs": st["npairs"], "c_blocks": st["nblocks_c"],
+             "tile": [st["tile_p"], st["tile_q"]],
+             "first_call_ms_incl_plan": first,
+             "unfold_panels": st["nunfold"],
+             "unfold": {"bytes": unf, "ms": prep_ms,
+                        "gbs": unf / prep_ms / 1e6 if prep_ms else None,
+                        "frac_of_8tbs": unf / prep_ms / 1e6 / HBM_PEAK
+                        if prep_ms else None},
+             "fold": {"bytes": st["cout_bytes"], "ms": st["post_ms"],
+                      "gbs": st["cout_bytes"] / st["post_ms"] / 1e6
+                      if st["post_ms"] else None}}
+        c2 = w.c.build(lib, alloc)
+        lib.fill_random(c2, args.seed * 16 + 7)
+        cb = canonical_bytes(w.c, esize)
+        add_ms = timed(lib, lambda: lib.add(1.0, c, 0.5, c2, "ijab", "ijab"),
+                       args.steps)
+        set_ms = timed(lib, lambda: lib.set(c, 0.0), args.steps)
+        f["xm_add"] = {"ms": add_ms, "bytes": 3 * cb,
+                       "gbs": 3 * cb / add_ms / 1e6,
+                       "frac_of_8tbs": 3 * cb / add_ms / 1e6 / HBM_PEAK}
+        f["xm_set"] = {"ms": set_ms, "bytes": cb, "gbs": cb / set_ms / 1e6,
+                       "frac_of_8tbs": cb / set_ms / 1e6 / HBM_PEAK}
+        log("ring %s: %.3f ms/call %.2f TFLOP/s (GEMM %.3f ms), unfold %.1f GB "
+            "%.3f ms, fold %.1f GB %.3f ms, add %.1f GB/s, set %.1f GB/s" % (
+                name, ms, F / ms / 1e9, gemm_ms, unf / 1e9, prep_ms,
+                st["cout_bytes"] / 1e9, st["post_ms"], f["xm_add"]["gbs"],
+                f["xm_set"]["gbs"]))
+        out["forms"][name] = f
+        for x in (t, v, c, c2):
+            lib.tensor_free(x)
+        lib.allocator_destroy(alloc)
+    print(json.dumps(out), flush=True)
+
+
+def precision_mode(lib, args):
+    """Ladder o=64 v=512, entries U[-1,1]*10^U[-6,0]: FP64, split (6 and 7
+    slices) and FP32, each timed and compared with the FP64 GPU result."""
+    from libxm_amd import xm, workloads
+    from libxm_amd.spec import rel_err
+    w = workloads.ladder_d2h(64, 512, args.seed, value_exp=(-6.0, 0.0))
+    alloc = lib.allocator()
+    t, v, c = (s.build(lib, alloc) for s in w.specs)
+    for k, x in enumerate((t, v, c)):
+        lib.fill_random(x, args.seed * 16 + k, -1.0, 1.0, -6.0, 0.0)
+        for ia, ib in workloads.antisym_strings(4, w.pairs):
+            lib.add(0.5, x, -0.5, x, ia, ib)
+    out = {"mode": "precision", "unit": "TFLOP/s (FP64-equivalent)",
+           "workload": "ladder o=64 v=512 D2h, U[-1,1]*10^U[-6,0]", "forms": {}}
+    c1 = w.c.build(lib, alloc)
+    results = {}
+    for name, prec, ns in (("fp64", xm.PRECISION_NATIVE, None),
+                           ("split6", xm.PRECISION_SPLIT_BF16, 6),
+                           ("split7", xm.PRECISION_SPLIT_BF16, 7)):
+        lib.set_precision(prec, ns)
+        ms, gemm_ms, prep_ms, st = time_steps(lib, w, t, v, c, args.steps,
+                                              max(args.warmup, 3))
+        lib.contract(w.alpha, t, v, 0.0, c1, *w.idx)
+        results[name] = w.c.read(lib, c1)
+        out["forms"][name] = {"ms_per_call": ms, "gemm_ms": gemm_ms,
+                              "slice_ms": prep_ms if ns else None,
+                              "tflops": st["flops"] / ms / 1e9}
+    lib.set_precision(xm.PRECISION_NATIVE)
+    F = st["flops"]
+    wf = workloads.ladder_d2h(64, 512, args.seed, type_=xm.SCALAR_FLOAT)
+    tf, vf, cf = (s.build(lib, alloc) for s in wf.specs)
+    for dst, src in ((tf, t), (vf, v), (cf, c)):
+        lib.copy(dst, 1.0, src, "abcd", "abcd")
+    ms, gemm_ms, _, _ = time_steps(lib, wf, tf, vf, cf, args.steps, args.warmup)
+    lib.contract(w.alpha, tf, vf, 0.0, cf, *w.idx)
+    results["fp32"] = wf.c.read(lib, cf)
+    out["forms"]["fp32"] = {"ms_per_call": ms, "gemm_ms": gemm_ms,
+                            "tflops": F / ms / 1e9}
+    for name in ("split6", "split7", "fp32"):
+        out["forms"][name]["rel_err_vs_fp64_gpu"] = rel_err(
+            results[name], results["fp64"])
+    out["flops"] = F
+    for name, f in out["forms"].items():
+        log("precision %s: %.3f ms/call %.2f TFLOP/s%s" % (
+            name, f["ms_per_call"], f["tflops"],
+            "" if name == "fp64" else ", rel err %.2e" % f["rel_err_vs_fp64_gpu"]))
+    print(json.dumps(out), flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary of
     this workload (profiles/traffic.json), or None."""
@@ -217,16 +341,21 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    # the first 3 calls of a plan tune its GEMM tile shape
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--o", type=int, default=64)
     ap.add_argument("--v", type=int, default=384)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--slices", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true")
-    ap.add_argument("--mode", choices=["contract", "copy"], default="contract",
+    ap.add_argument("--mode", choices=["contract", "copy", "ring", "precision"],
+                    default="contract",
                     help="copy: xm_copy of a 4 GiB FP64 tensor, a plain "
-                         "copy of known size (HBM counter calibration)")
+                         "copy of known size (HBM counter calibration); "
+                         "ring: the o=96 v=480 ring in FP64 and complex64 "
+                         "with xm_add / xm_set; precision: the o=64 v=512 "
+                         "ladder over six decades in FP64, split and FP32")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build runs on a single "
@@ -237,6 +366,10 @@ def main():
     lib = xm.load()
     if args.mode == "copy":
         return copy_calibration(lib, args.steps)
+    if args.mode == "ring":
+        return ring_mode(lib, args)
+    if args.mode == "precision":
+        return precision_mode(lib, args)
     w = workloads.ladder_d2h(args.o, args.v, args.seed)
     log("workload", json.dumps(w.describe()))
     tb = time.perf_counter()
@@ -273,7 +406,9 @@ def main():
                      "peak": FP64_PEAK, "unit": "TFLOP/s",
                      "frac": F / (gemm_ms * 1e-3) / 1e12 / FP64_PEAK,
                      "traffic": measured_traffic("k_gemm_f64"),
-                     "kernel": "k_gemm_f64", "kernel_ms": gemm_ms},
+                     "kernel": "k_gemm_f64", "kernel_ms": gemm_ms,
+                     "tile": [st["tile_p"], st["tile_q"]],
+                     "tile_tuning_in_timed_steps": bool(st["tile_tuning"])},
     }
     if not args.no_split:
         lib.set_precision(xm.PRECISION_SPLIT_BF16, args.slices)

@@ -59,6 +59,14 @@ typedef struct {
 	int precision;		/* xm_precision_t used */
 	int nslices;		/* slices per operand (SPLIT_BF16 only) */
 	int gemm_launches;	/* number of grouped GEMM launches */
+	/* unfold_bytes by stage: operand panels and C-in permutes run inside
+	 * prep_ms (the C-in permute only when beta != 0), C-out inside post_ms */
+	double panel_bytes;	/* operand panels, read + written */
+	double cin_bytes;	/* permuted C in, read + written */
+	double cout_bytes;	/* permuted C out, read + written */
+	/* output tile of the grouped GEMM launch, and 1 while the plan's tile
+	 * shape was still being tuned (the call timed one candidate) */
+	int tile_p, tile_q, tile_tuning;
 } xm_contract_stats_t;
 
 void xm_contract_get_stats(xm_contract_stats_t *stats);

@@ -83,6 +83,9 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 	acc->prep_ms += st.prep_ms;
 	acc->post_ms += st.post_ms;
 	acc->gemm_launches += st.gemm_launches;
+	acc->panel_bytes += st.panel_bytes;
+	acc->cin_bytes += st.cin_bytes;
+	acc->cout_bytes += st.cout_bytes;
 	acc->precision = st.precision;
 	acc->nslices = st.nslices;
 	(void)x_first;

@@ -357,6 +357,7 @@ struct plan_builder {
 		}
 		pl->unfold_jobs.push_back(j);
 		pl->unfold_bytes += 2.0 * P * K * esz;
+		pl->panel_bytes += 2.0 * P * K * esz;
 		*ptr = off;
 		*in_ws = true;
 	}
@@ -463,6 +464,8 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 			pl->cin_jobs.push_back(in);
 			pl->cout_jobs.push_back(out);
 			pl->unfold_bytes += 4.0 * m * n * pb.esz;
+			pl->cin_bytes += 2.0 * m * n * pb.esz;
+			pl->cout_bytes += 2.0 * m * n * pb.esz;
 			cb.c = off;
 			cb.flags |= 1;
 		} else {
@@ -618,7 +621,9 @@ plan_upload(contract_plan *pl)
 	add(nullptr, sizeof(gemm_scalars));
 	add(pl->cblocks.data(), pl->cblocks.size() * sizeof(gemm_cblock));
 	add(pl->pairs.data(), pl->pairs.size() * sizeof(gemm_pair));
-	add(pl->tiles.data(), pl->tiles.size() * sizeof(gemm_tile));
+	for (int c = 0; c < pl->ncand; c++)
+		add(pl->cand_tiles[c].data(),
+		    pl->cand_tiles[c].size() * sizeof(gemm_tile));
 	std::vector<char> ju = elem_jobs_pack(pl->unfold_jobs);
 	std::vector<char> ji = elem_jobs_pack(pl->cin_jobs);
 	std::vector<char> jo = elem_jobs_pack(pl->cout_jobs);
@@ -643,10 +648,13 @@ plan_upload(contract_plan *pl)
 	pl->d_scalars = (gemm_scalars *)(base + offs[0]);
 	pl->d_cblocks = (gemm_cblock *)(base + offs[1]);
 	pl->d_pairs = (gemm_pair *)(base + offs[2]);
-	pl->d_tiles = (gemm_tile *)(base + offs[3]);
-	pl->d_unfold = base + offs[4];
-	pl->d_cin = base + offs[5];
-	pl->d_cout = base + offs[6];
+	for (int c = 0; c < pl->ncand; c++)
+		pl->d_cand_tiles[c] = (gemm_tile *)(base + offs[3 + c]);
+	const size_t oj = 3 + (size_t)pl->ncand;
+	pl->d_unfold = base + offs[oj];
+	pl->d_cin = base + offs[oj + 1];
+	pl->d_cout = base + offs[oj + 2];
+	select_tiles(pl);
 	HIP_OK(fn, hipHostMalloc(&pl->host_scalars, sizeof(gemm_scalars),
 	    hipHostMallocDefault));
 	for (auto &e : pl->ev)
@@ -784,6 +792,13 @@ xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
 	st.precision = pl->precision;
 	st.nslices = pl->nslices;
 	st.gemm_launches = pl->gemm_launches;
+	st.panel_bytes = pl->panel_bytes;
+	st.cin_bytes = pl->cin_bytes;
+	st.cout_bytes = pl->cout_bytes;
+	st.tile_p = (int)pl->cand_bp[pl->tile_active];
+	st.tile_q = (int)pl->cand_bq[pl->tile_active];
+	st.tile_tuning = pl->tile_choice < 0;
+	gemm_tile_record(pl, (float)st.gemm_ms);
 }
 
 XM_API void

@@ -7,7 +7,10 @@
 
 #include "internal.h"
 
+#include <string>
 #include <vector>
+
+#define MAX_TILE_CANDIDATES 2
 
 struct masks_t {
 	xm_dim_t cidxa, aidxa, cidxb, aidxb, cidxc, aidxc;
@@ -37,6 +40,7 @@ struct contract_plan {
 
 	/* work */
 	double flops = 0, flops_pre = 0, unfold_bytes = 0;
+	double panel_bytes = 0, cin_bytes = 0, cout_bytes = 0;
 	size_t npairs = 0, npairs_pre = 0, nblocks_c = 0, nunfold = 0;
 	size_t nsubstituted = 0;
 	std::vector<plan_pair> pair_list;
@@ -46,9 +50,21 @@ struct contract_plan {
 	std::vector<bool> cpanel;
 	std::vector<gemm_pair> pairs;
 	std::vector<bool> x_in_ws, y_in_ws, pair_swap;
-	std::vector<gemm_tile> tiles;
+	std::vector<gemm_tile> tiles;	/* the active candidate's */
 	int tile_cfg = 0;
 	int gemm_launches = 0;
+
+	/* tile-shape candidates (gemm.hip autotuner): candidate c launches
+	 * cand_tiles[c] (device copy d_cand_tiles[c]) with cand_bp/bq[c] */
+	int ncand = 0;
+	std::vector<gemm_tile> cand_tiles[MAX_TILE_CANDIDATES];
+	gemm_tile *d_cand_tiles[MAX_TILE_CANDIDATES] = {};
+	uint32_t cand_bp[MAX_TILE_CANDIDATES] = {}, cand_bq[MAX_TILE_CANDIDATES] = {};
+	float cand_ms[MAX_TILE_CANDIDATES] = {};
+	int tile_choice = -1;		/* decided candidate, -1 while tuning */
+	int tile_next = 0;		/* calls of the tuning sequence so far */
+	int tile_active = 0;		/* candidate of the last launch */
+	std::string tile_sig;		/* block-size histogram signature */
 
 	/* permute jobs: operand panels, C in, C out */
 	std::vector<elem_job> unfold_jobs, cin_jobs, cout_jobs;
@@ -82,7 +98,9 @@ void contract_set_stats(const xm_contract_stats_t &s);
 
 /* gemm.hip */
 void build_tiles(contract_plan *pl);
+void select_tiles(contract_plan *pl);
 void launch_gemm(contract_plan *pl, hipStream_t s);
+void gemm_tile_record(contract_plan *pl, float gemm_ms);
 
 /* split.hip */
 void split_prepare(contract_plan *pl, const xm_tensor_t *a,

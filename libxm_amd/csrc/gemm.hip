@@ -17,7 +17,12 @@
 #include "contract_plan.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -327,38 +332,82 @@ use_simple(void)
 	return v;
 }
 
-/* Output tile (P x Q) of the kernel the plan will launch. */
+/*
+ * Tile-shape candidates of a plan.  The FP64 kernel is compiled for
+ * 64 x 128 and 64 x 64 output tiles: 64 x 128 moves fewer operand bytes
+ * per FLOP, 64 x 64 wastes less on small C blocks (P, Q of a few 16s).  The
+ * first calls of a plan time each candidate once (the GEMM's own HIP
+ * events); later calls launch the faster one.  Decisions are also kept per
+ * block-size histogram signature, so a new plan whose C blocks have the
+ * same shape distribution starts with the decision.  Every candidate gives
+ * bit-identical results: each output element accumulates over the same k
+ * sequence in the same order.  XM_GEMM_TILE=64x128 or 64x64 pins a shape.
+ * The split form has one shape per slice count; FP32 and complex one.
+ */
 static void
-tile_shape(const contract_plan *pl, uint32_t *bp, uint32_t *bq)
+tile_candidates(const contract_plan *pl, std::vector<std::pair<uint32_t,
+    uint32_t>> &out)
 {
-	*bp = 64;
-	*bq = 64;
 	if (pl->precision == XM_PRECISION_SPLIT_BF16) {
-		if (split_staged_kernel(pl->nslices)) {
-			*bp = 128;
-			*bq = 128;
-		}
+		out.push_back(split_staged_kernel(pl->nslices) ?
+		    std::make_pair(128u, 128u) : std::make_pair(64u, 64u));
 	} else if (pl->type == XM_SCALAR_DOUBLE && !use_simple()) {
-		*bq = 128;
+		const char *e = getenv("XM_GEMM_TILE");
+		if (e && strcmp(e, "64x64") == 0)
+			out.push_back({64, 64});
+		else if (e && strcmp(e, "64x128") == 0)
+			out.push_back({64, 128});
+		else {
+			out.push_back({64, 128});
+			out.push_back({64, 64});
+		}
+	} else {
+		out.push_back({64, 64});
 	}
 }
 
+/* decided candidate per block-size histogram signature */
+static std::map<std::string, int> g_tile_choice;
+
 /*
- * Work tiles in launch order.  Workgroup w of a launch runs on XCD w % 8
- * (round-robin dispatch) and each XCD has its own L2, so the tiles of one
- * C block are kept on one XCD: C blocks are spread over the 8 XCDs by a
- * greedy longest-first balance of their work, every XCD walks its C blocks
- * with the P-tiles innermost (tiles running together share the Y panel
- * columns and the X panel), and slot s of XCD x becomes workgroup
+ * Signature: element type, precision, and the count of C blocks (rounded
+ * to a power of two) per (log2 P, log2 Q, log2 K) bucket.
+ */
+static std::string
+tile_signature(const contract_plan *pl)
+{
+	std::map<std::tuple<int, int, int>, size_t> h;
+	auto lg = [](double x) { return x < 1 ? 0 : (int)std::log2(x); };
+	for (const gemm_cblock &cb : pl->cblocks) {
+		double kt = 0;
+		for (uint32_t p = cb.pair_begin; p < cb.pair_end; p++)
+			kt += pl->pairs[p].k;
+		h[std::make_tuple(lg(cb.p), lg(cb.q), lg(kt))]++;
+	}
+	std::string sig = std::to_string((int)pl->type) + "/" +
+	    std::to_string(pl->precision) + "/" + std::to_string(pl->nslices);
+	for (auto &e : h)
+		sig += ":" + std::to_string(std::get<0>(e.first)) + "," +
+		    std::to_string(std::get<1>(e.first)) + "," +
+		    std::to_string(std::get<2>(e.first)) + "x" +
+		    std::to_string(lg((double)e.second));
+	return sig;
+}
+
+/*
+ * Work tiles of one shape in launch order.  Workgroup w of a launch runs on
+ * XCD w % 8 (round-robin dispatch) and each XCD has its own L2, so the
+ * tiles of one C block are kept on one XCD: C blocks are spread over the 8
+ * XCDs by a greedy longest-first balance of their work, every XCD walks its
+ * C blocks with the P-tiles innermost (tiles running together share the Y
+ * panel columns and the X panel), and slot s of XCD x becomes workgroup
  * 8 * s + x.  Shorter XCD lists are padded with empty tiles.
  */
-void
-build_tiles(contract_plan *pl)
+static void
+order_tiles(const contract_plan *pl, uint32_t BP, uint32_t BQ,
+    std::vector<gemm_tile> &tiles)
 {
 	const int NXCD = 8;
-	uint32_t BP, BQ;
-	tile_shape(pl, &BP, &BQ);
-	pl->tile_cfg = (int)(BP * 1000 + BQ);
 	std::vector<std::pair<double, uint32_t>> order;
 	for (uint32_t i = 0; i < pl->cblocks.size(); i++) {
 		const gemm_cblock &cb = pl->cblocks[i];
@@ -384,10 +433,70 @@ build_tiles(contract_plan *pl)
 	size_t per = 0;
 	for (auto &b : bins)
 		per = std::max(per, b.size());
-	pl->tiles.assign(per * NXCD, gemm_tile{0xffffffffu, 0, 0, 0});
+	tiles.assign(per * NXCD, gemm_tile{0xffffffffu, 0, 0, 0});
 	for (int x = 0; x < NXCD; x++)
 		for (size_t s = 0; s < bins[x].size(); s++)
-			pl->tiles[s * NXCD + x] = bins[x][s];
+			tiles[s * NXCD + x] = bins[x][s];
+}
+
+void
+build_tiles(contract_plan *pl)
+{
+	std::vector<std::pair<uint32_t, uint32_t>> cand;
+	tile_candidates(pl, cand);
+	pl->ncand = (int)std::min<size_t>(cand.size(), MAX_TILE_CANDIDATES);
+	for (int c = 0; c < pl->ncand; c++) {
+		pl->cand_bp[c] = cand[c].first;
+		pl->cand_bq[c] = cand[c].second;
+		order_tiles(pl, cand[c].first, cand[c].second, pl->cand_tiles[c]);
+		pl->cand_ms[c] = -1.0f;
+	}
+	pl->tile_sig = tile_signature(pl);
+	pl->tile_next = 0;
+	pl->tile_choice = pl->ncand == 1 ? 0 : -1;
+	auto it = g_tile_choice.find(pl->tile_sig);
+	if (pl->tile_choice < 0 && it != g_tile_choice.end())
+		pl->tile_choice = it->second;
+}
+
+/* Make the candidate to launch next the plan's active tile list (the
+ * decided one, or candidate 0 for the plan's first, untimed call). */
+void
+select_tiles(contract_plan *pl)
+{
+	int c = pl->tile_choice >= 0 ? pl->tile_choice : 0;
+	pl->tile_active = c;
+	pl->tiles = pl->cand_tiles[c];
+	pl->d_tiles = pl->d_cand_tiles[c];
+	pl->tile_cfg = (int)(pl->cand_bp[c] * 1000 + pl->cand_bq[c]);
+}
+
+/* Time of the last launch (candidate tile_active); decide once every
+ * candidate has one.  The plan's first call runs candidate 0 untimed: it
+ * pays for cold caches and first launches, which would bias the choice.
+ * tile_next counts calls: call 0 warms, call c + 1 times candidate c. */
+void
+gemm_tile_record(contract_plan *pl, float gemm_ms)
+{
+	if (pl->tile_choice >= 0)
+		return;
+	if (pl->tile_next > 0)
+		pl->cand_ms[pl->tile_active] = gemm_ms;
+	if (++pl->tile_next <= pl->ncand) {
+		pl->tile_active = pl->tile_next - 1;
+		pl->tiles = pl->cand_tiles[pl->tile_active];
+		pl->d_tiles = pl->d_cand_tiles[pl->tile_active];
+		pl->tile_cfg = (int)(pl->cand_bp[pl->tile_active] * 1000 +
+		    pl->cand_bq[pl->tile_active]);
+		return;
+	}
+	int best = 0;
+	for (int c = 1; c < pl->ncand; c++)
+		if (pl->cand_ms[c] < pl->cand_ms[best])
+			best = c;
+	pl->tile_choice = best;
+	g_tile_choice[pl->tile_sig] = best;
+	select_tiles(pl);
 }
 
 void
@@ -403,8 +512,11 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 		if (use_simple())
 			k_gemm_simple<double><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
-		else
+		else if (pl->cand_bq[pl->tile_active] == 128)
 			k_gemm_f64<64, 128, 16><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		else
+			k_gemm_f64<64, 64, 16><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_FLOAT:

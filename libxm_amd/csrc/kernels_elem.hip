@@ -88,24 +88,34 @@ template <> __device__ __forceinline__ cplx<double> conv(cplx<double> x) { retur
 
 /* ---- chunk -> job lookup ------------------------------------------ */
 
+/*
+ * The last job with first_chunk <= chunk (jobs[0].first_chunk is 0).  Each
+ * round every thread tests one job of the remaining range at a stride of
+ * range / WG and the range shrinks to the bucket found: two rounds of one
+ * load per thread for up to WG^2 jobs, where a one-thread binary search
+ * would chain log2(njobs) dependent loads.  All threads must call it.
+ */
 template <typename J>
 __device__ __forceinline__ uint32_t
 find_job(const J *jobs, uint32_t njobs, uint64_t chunk)
 {
-	__shared__ uint32_t found;
-	if (threadIdx.x == 0) {
-		uint32_t lo = 0, hi = njobs - 1;
-		while (lo < hi) {
-			uint32_t mid = (lo + hi + 1) / 2;
-			if (jobs[mid].first_chunk <= chunk)
-				lo = mid;
-			else
-				hi = mid - 1;
-		}
-		found = lo;
+	__shared__ uint32_t best;
+	uint32_t lo = 0, hi = njobs;	/* answer in [lo, hi) */
+	while (hi - lo > 1) {
+		const uint32_t stride = (hi - lo + WG - 1) / WG;
+		if (threadIdx.x == 0)
+			best = 0;
+		__syncthreads();
+		const uint32_t j = lo + threadIdx.x * stride;
+		if (j < hi && jobs[j].first_chunk <= chunk)
+			atomicMax(&best, threadIdx.x);
+		__syncthreads();
+		const uint32_t b = best;
+		__syncthreads();
+		lo += b * stride;
+		hi = min(hi, lo + stride);
 	}
-	__syncthreads();
-	return found;
+	return lo;
 }
 
 /* ---- fills -------------------------------------------------------- */
@@ -394,11 +404,28 @@ to_dev(const elem_job *jobs, size_t n)
 		b.src = a.src;
 		b.n = a.n;
 		b.first_chunk = a.first_chunk;
-		b.ndim = a.ndim;
 		b.src_zero = a.src_zero;
+		/* merge dimension d + 1 into d where the source walks them as
+		 * one run (stride[d+1] = stride[d] * dims[d]): an identity
+		 * permutation becomes one dimension, one multiply per element */
+		uint32_t dims[XM_ELEM_MAXDIM];
+		uint64_t str[XM_ELEM_MAXDIM];
+		uint32_t nd = 0;
+		for (uint32_t d = 0; d < a.ndim; d++) {
+			if (a.dims[d] == 1)
+				continue;	/* contributes offset 0 */
+			if (nd > 0 && str[nd - 1] * dims[nd - 1] == a.sstride[d] &&
+			    (uint64_t)dims[nd - 1] * a.dims[d] <= 0xffffffffu) {
+				dims[nd - 1] *= a.dims[d];
+				continue;
+			}
+			dims[nd] = a.dims[d];
+			str[nd++] = a.sstride[d];
+		}
+		b.ndim = nd;
 		for (uint32_t d = 0; d < XM_ELEM_MAXDIM; d++) {
-			b.div[d] = make_fdiv(d < a.ndim && a.dims[d] ? a.dims[d] : 1);
-			b.sstride[d] = d < a.ndim ? a.sstride[d] : 0;
+			b.div[d] = make_fdiv(d < nd && dims[d] ? dims[d] : 1);
+			b.sstride[d] = d < nd ? str[d] : 0;
 		}
 		b.f_re = a.f_re;
 		b.f_im = a.f_im;

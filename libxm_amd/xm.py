@@ -61,7 +61,10 @@ class ContractStats(C.Structure):
                 ("prep_ms", C.c_double), ("post_ms", C.c_double),
                 ("total_ms", C.c_double), ("plan_cached", C.c_int),
                 ("precision", C.c_int), ("nslices", C.c_int),
-                ("gemm_launches", C.c_int)]
+                ("gemm_launches", C.c_int), ("panel_bytes", C.c_double),
+                ("cin_bytes", C.c_double), ("cout_bytes", C.c_double),
+                ("tile_p", C.c_int), ("tile_q", C.c_int),
+                ("tile_tuning", C.c_int)]
 
 
 def dim(seq):
