@@ -37,6 +37,17 @@ void xm_set_split_slices(int nslices);
 int xm_get_split_slices(void);
 
 /*
+ * Graph launch (no libxm counterpart).  Once a contraction plan's GEMM tile
+ * shape is decided, its device work is captured as a hipGraph and each
+ * later call with that plan is one graph launch; alpha and beta are read
+ * from device memory written before the launch.  On by default; 0 turns it
+ * off (plans then launch their kernels one by one).  The environment
+ * variable XM_GRAPH=0 sets the default to off.  Results are identical.
+ */
+void xm_set_graph_launch(int on);
+int xm_get_graph_launch(void);
+
+/*
  * Figures of the most recent xm_contract / xm_contract_backward call.
  * No libxm counterpart (libxm has no timers or counters).
  * flops counts 2*m*n*k real flops (x4 for complex types) over the block
@@ -67,6 +78,7 @@ typedef struct {
 	/* output tile of the grouped GEMM launch, and 1 while the plan's tile
 	 * shape was still being tuned (the call timed one candidate) */
 	int tile_p, tile_q, tile_tuning;
+	int graph;		/* 1 if the call replayed the plan's hipGraph */
 } xm_contract_stats_t;
 
 void xm_contract_get_stats(xm_contract_stats_t *stats);

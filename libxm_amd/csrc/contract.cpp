@@ -45,6 +45,31 @@ xm_get_contract_precision(void)
 	return (xm_precision_t)g_precision;
 }
 
+/* graph launch of decided plans: on unless XM_GRAPH=0 or switched off */
+static int g_graph = -1;
+
+bool
+graph_launch_enabled(void)
+{
+	if (g_graph < 0) {
+		const char *e = getenv("XM_GRAPH");
+		g_graph = !(e && *e == '0');
+	}
+	return g_graph != 0;
+}
+
+XM_API void
+xm_set_graph_launch(int on)
+{
+	g_graph = on != 0;
+}
+
+XM_API int
+xm_get_graph_launch(void)
+{
+	return graph_launch_enabled();
+}
+
 XM_API void
 xm_set_split_slices(int n)
 {
@@ -582,6 +607,9 @@ plan_release(contract_plan *pl)
 		HIP_OK("xm_contract", hipFree(pl->ws));
 	if (pl->host_scalars)
 		HIP_OK("xm_contract", hipHostFree(pl->host_scalars));
+	for (auto &g : pl->graph)
+		if (g)
+			HIP_OK("xm_contract", hipGraphExecDestroy(g));
 	if (pl->ev[0])
 		for (auto &e : pl->ev)
 			HIP_OK("xm_contract", hipEventDestroy(e));
@@ -723,6 +751,9 @@ plan_cache_drop(uint64_t id)
 
 static xm_contract_stats_t last_stats;
 
+static void launch_sequence(contract_plan *pl, uint32_t flags, hipStream_t s,
+    bool stamps);
+
 static void
 run_plan(contract_plan *pl, xm_scalar_t alpha, xm_scalar_t beta)
 {
@@ -740,27 +771,79 @@ run_plan(contract_plan *pl, xm_scalar_t alpha, xm_scalar_t beta)
 	*pl->host_scalars = sc;
 	HIP_OK(fn, hipMemcpyAsync(pl->d_scalars, pl->host_scalars, sizeof sc,
 	    hipMemcpyHostToDevice, s));
-	bool do_pairs = !(sc.flags & 1);
-	HIP_OK(fn, hipEventRecord(pl->ev[0], s));
+	/*
+	 * Once the tile shape is decided the device sequence is captured as a
+	 * graph (per alpha/beta-zero combination, which select the launches)
+	 * and replayed: one launch per call.  alpha and beta are read from
+	 * d_scalars, written above outside the graph, so a graph serves every
+	 * call.  The sequence holds kernels and event records only (no
+	 * memset nodes).  A capture that fails leaves the plan on direct
+	 * launches.
+	 */
+	pl->last_graph = false;
+	hipGraphExec_t &g = pl->graph[sc.flags & 3];
+	if (graph_launch_enabled() && pl->tile_choice >= 0 && !pl->graph_failed &&
+	    g == nullptr) {
+		hipGraph_t graph = nullptr;
+		bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) ==
+		    hipSuccess;
+		if (ok)
+			launch_sequence(pl, sc.flags, s, true);
+		ok = hipStreamEndCapture(s, &graph) == hipSuccess && ok &&
+		    graph != nullptr;
+		if (ok)
+			ok = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0) ==
+			    hipSuccess;
+		if (graph)
+			(void)hipGraphDestroy(graph);
+		if (!ok) {
+			(void)hipGetLastError();
+			g = nullptr;
+			pl->graph_failed = true;
+		}
+	}
+	if (graph_launch_enabled() && g) {
+		HIP_OK(fn, hipGraphLaunch(g, s));
+		pl->last_graph = true;
+	} else {
+		launch_sequence(pl, sc.flags, s, false);
+	}
+	HIP_OK(fn, hipStreamSynchronize(s));
+}
+
+/* The plan's device work for one call: permutes in, slices, grouped GEMM,
+ * permutes out, with four time marks between the stages: HIP events, or
+ * (inside a graph, where events cannot be timed) device wall-clock stamps. */
+static void
+launch_sequence(contract_plan *pl, uint32_t flags, hipStream_t s, bool stamps)
+{
+	const char *fn = "xm_contract";
+	bool do_pairs = !(flags & 1);
+	auto mark = [&](int i) {
+		if (stamps)
+			launch_stamp(stamp_slots() + i, s);
+		else
+			HIP_OK(fn, hipEventRecord(pl->ev[i], s));
+	};
+	mark(0);
 	if (do_pairs && pl->unfold_chunks)
 		launch_elem_packed(pl->d_unfold, pl->unfold_jobs.size(),
 		    pl->unfold_chunks, ELEM_GATHER, pl->type, pl->type, 0, s);
-	if (!(sc.flags & 2) && pl->cin_chunks)
+	if (!(flags & 2) && pl->cin_chunks)
 		launch_elem_packed(pl->d_cin, pl->cin_jobs.size(),
 		    pl->cin_chunks, ELEM_GATHER, pl->type, pl->type, 0, s);
 	if (do_pairs && pl->precision == XM_PRECISION_SPLIT_BF16)
 		split_slices(pl, s);
-	HIP_OK(fn, hipEventRecord(pl->ev[1], s));
+	mark(1);
 	if (pl->precision == XM_PRECISION_SPLIT_BF16)
 		split_gemm(pl, s);
 	else
 		launch_gemm(pl, s);
-	HIP_OK(fn, hipEventRecord(pl->ev[2], s));
+	mark(2);
 	if (pl->cout_chunks)
 		launch_elem_packed(pl->d_cout, pl->cout_jobs.size(),
 		    pl->cout_chunks, ELEM_GATHER, pl->type, pl->type, 0, s);
-	HIP_OK(fn, hipEventRecord(pl->ev[3], s));
-	HIP_OK(fn, hipStreamSynchronize(s));
+	mark(3);
 }
 
 XM_API void
@@ -784,9 +867,16 @@ xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
 	st.nblocks_c = pl->nblocks_c;
 	st.nunfold = pl->nunfold;
 	st.unfold_bytes = pl->unfold_bytes;
-	st.prep_ms = elapsed_ms(pl->ev[0], pl->ev[1]);
-	st.gemm_ms = elapsed_ms(pl->ev[1], pl->ev[2]);
-	st.post_ms = elapsed_ms(pl->ev[2], pl->ev[3]);
+	if (pl->last_graph) {
+		const uint64_t *t = stamp_slots();
+		st.prep_ms = stamp_ms(t[0], t[1]);
+		st.gemm_ms = stamp_ms(t[1], t[2]);
+		st.post_ms = stamp_ms(t[2], t[3]);
+	} else {
+		st.prep_ms = elapsed_ms(pl->ev[0], pl->ev[1]);
+		st.gemm_ms = elapsed_ms(pl->ev[1], pl->ev[2]);
+		st.post_ms = elapsed_ms(pl->ev[2], pl->ev[3]);
+	}
 	st.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
 	st.plan_cached = cached;
 	st.precision = pl->precision;
@@ -798,6 +888,7 @@ xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
 	st.tile_p = (int)pl->cand_bp[pl->tile_active];
 	st.tile_q = (int)pl->cand_bq[pl->tile_active];
 	st.tile_tuning = pl->tile_choice < 0;
+	st.graph = pl->last_graph;
 	gemm_tile_record(pl, (float)st.gemm_ms);
 }
 

@@ -82,6 +82,12 @@ struct contract_plan {
 	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
 	split_state *split = nullptr;
+
+	/* the device launch sequence captured as a graph, one per (alpha == 0,
+	 * beta == 0) combination; graph_failed after a capture error */
+	hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+	bool graph_failed = false;
+	bool last_graph = false;	/* the last call replayed a graph */
 };
 
 void contract_masks(const char *fn, const xm_tensor_t *a,
@@ -91,6 +97,7 @@ size_t enumerate_pairs(const xm_tensor_t *a, const xm_tensor_t *b,
     const xm_dim_t &blkc, const masks_t &mk, std::vector<block_pair> &all,
     std::vector<std::vector<uint32_t>> *members);
 int current_precision(void);
+bool graph_launch_enabled(void);
 int current_nslices(void);
 void plan_cache_clear(void);
 void plan_cache_drop(uint64_t tensor_id);

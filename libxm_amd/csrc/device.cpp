@@ -94,6 +94,39 @@ elapsed_ms(hipEvent_t a, hipEvent_t b)
 	return ms;
 }
 
+/*
+ * Stage timestamps of graph-launched plans (events recorded inside a graph
+ * cannot be timed): pinned host slots the stamp kernel writes the device
+ * wall clock into, and that clock's rate.  Calls are synchronous, so every
+ * plan's graph can share the slots.
+ */
+uint64_t *
+stamp_slots(void)
+{
+	static uint64_t *slots = nullptr;
+	if (slots == nullptr) {
+		HIP_OK("libxm", hipHostMalloc((void **)&slots,
+		    STAMP_SLOTS * sizeof(uint64_t), hipHostMallocMapped));
+		memset(slots, 0, STAMP_SLOTS * sizeof(uint64_t));
+	}
+	return slots;
+}
+
+double
+stamp_ms(uint64_t a, uint64_t b)
+{
+	static int khz = 0;
+	if (khz == 0) {
+		int dev = 0;
+		HIP_OK("libxm", hipGetDevice(&dev));
+		HIP_OK("libxm", hipDeviceGetAttribute(&khz,
+		    hipDeviceAttributeWallClockRate, dev));
+		if (khz <= 0)
+			XM_FATAL("libxm", "device wall clock rate unavailable");
+	}
+	return b >= a ? (double)(b - a) / khz : 0.0;
+}
+
 XM_API int
 xm_amd_synchronize(void)
 {

@@ -112,6 +112,11 @@ void *scratch_buffer(int slot, size_t bytes);
 /* Upload a host vector into a device scratch slot (async on xm_stream). */
 void *upload(int slot, const void *host, size_t bytes);
 double elapsed_ms(hipEvent_t a, hipEvent_t b);
+#define STAMP_SLOTS 4
+uint64_t *stamp_slots(void);
+double stamp_ms(uint64_t a, uint64_t b);
+/* kernels_elem.hip: one thread stores the device wall clock in *slot */
+void launch_stamp(uint64_t *slot, hipStream_t s);
 
 /* ---- kernel launchers (*.hip) -------------------------------------- */
 

@@ -510,6 +510,22 @@ launch_elem(const elem_job *jobs, size_t njobs, size_t nchunks, int op,
 	launch_elem_packed(dj, njobs, nchunks, op, dt, st, alpha, s);
 }
 
+/* The device wall clock (constant rate, hipDeviceAttributeWallClockRate
+ * kHz) when the stream reaches this launch. */
+__global__ void
+k_stamp(uint64_t *slot)
+{
+	if (threadIdx.x == 0)
+		*slot = wall_clock64();
+}
+
+void
+launch_stamp(uint64_t *slot, hipStream_t s)
+{
+	k_stamp<<<1, 64, 0, s>>>(slot);
+	check_launch("libxm stamp kernel");
+}
+
 void
 launch_dot(const elem_job *jobs, size_t njobs, size_t nchunks,
     xm_scalar_type_t type, double *dpartials, hipStream_t s)

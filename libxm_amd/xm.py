@@ -64,7 +64,7 @@ class ContractStats(C.Structure):
                 ("gemm_launches", C.c_int), ("panel_bytes", C.c_double),
                 ("cin_bytes", C.c_double), ("cout_bytes", C.c_double),
                 ("tile_p", C.c_int), ("tile_q", C.c_int),
-                ("tile_tuning", C.c_int)]
+                ("tile_tuning", C.c_int), ("graph", C.c_int)]
 
 
 def dim(seq):
@@ -181,6 +181,8 @@ _SIGNATURES = {
     "xm_get_contract_precision": (C.c_int, []),
     "xm_set_split_slices": (None, [C.c_int]),
     "xm_get_split_slices": (C.c_int, []),
+    "xm_set_graph_launch": (None, [C.c_int]),
+    "xm_get_graph_launch": (C.c_int, []),
     "xm_contract_backward": (None, [Scalar, _P, _P, _P, _P, _P, _S, _S, _S]),
     "xm_amd_fill_random": (None, [_P, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double]),
     "xm_amd_allocator_usage": (None, [_P, C.POINTER(_SZ), C.POINTER(_SZ)]),
@@ -195,6 +197,7 @@ CORE_SYMBOLS = [n for n in _SIGNATURES
                               "xm_set_contract_precision",
                               "xm_get_contract_precision",
                               "xm_set_split_slices", "xm_get_split_slices",
+                              "xm_set_graph_launch", "xm_get_graph_launch",
                               "xm_contract_backward")]
 
 _libc = C.CDLL(None)
@@ -380,6 +383,9 @@ class XmLib:
         self.lib.xm_set_contract_precision(int(precision))
         if nslices is not None:
             self.lib.xm_set_split_slices(int(nslices))
+
+    def set_graph_launch(self, on):
+        self.lib.xm_set_graph_launch(1 if on else 0)
 
     def fill_random(self, t, seed, lo=-1.0, hi=1.0, wlo=0.0, whi=0.0):
         self.lib.xm_amd_fill_random(t, int(seed), lo, hi, wlo, whi)
