@@ -64,6 +64,7 @@ struct contract_plan {
 	int tile_choice = -1;		/* decided candidate, -1 while tuning */
 	int tile_next = 0;		/* calls of the tuning sequence so far */
 	int tile_active = 0;		/* candidate of the last launch */
+	bool f64_dma = false;		/* k_gemm_f64_dma can run the 64 x 128 tiles */
 	std::string tile_sig;		/* block-size histogram signature */
 
 	/* permute jobs: operand panels, C in, C out */

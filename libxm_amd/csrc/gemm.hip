@@ -196,6 +196,200 @@ k_gemm_f64(const gemm_tile *__restrict__ tiles,
 			}
 }
 
+/* ---- FP64, operands staged by DMA -------------------------------- */
+
+/*
+ * FP64 tile kernel for plans whose every X and Y panel is P-contiguous
+ * with an even row count and leading dimension and a 16-byte aligned base
+ * (the planner checks): 64 x 128 outputs, 4 waves as 2 x 2 of 32 x 64.
+ * Per 16-k step the tiles move global -> LDS by global_load_lds_dwordx4,
+ * lane l carrying rows 2(l % 32), +1 of one column: an X DMA holds two
+ * 64-row columns, a Y DMA one 128-row column, laid out [k][row]; columns
+ * past K and row pairs past the panel read a zero page.  1.5 BK DMAs per
+ * step over the 4 waves; NST rotating stages with the DMA NST - 1 steps
+ * ahead, a counted vmcnt wait and a bare s_barrier per step, and the
+ * stages reach the step as restrict pointers (see k_gemm_split2 for why
+ * each matters).  MINW (waves per SIMD the compiler must allow for) keeps
+ * the accumulators in VGPRs: without it they go to AGPRs and the kernel
+ * ran 13% slower.  On the o=64 v=384 ladder BK 16, 2 stages, 3 waves per
+ * SIMD (18.4 ms) edged out BK 16 / 3 stages / 2 waves (18.7), BK 8 with
+ * 2-4 stages at 3-5 waves (18.7-19.0).  The pair
+ * factor al scales X values in registers, after the LDS read.  Same
+ * summation order per output as k_gemm_f64: results are identical.
+ */
+#define FD_BP 64
+#define FD_BQ 128
+
+typedef __attribute__((address_space(3))) void fd_lds_t;
+typedef __attribute__((address_space(1))) void fd_gbl_t;
+
+static __device__ uint4 fd_zero_page[64];
+
+template <int FD_BK>
+struct fd_stage {
+	double x[FD_BK * FD_BP];	/* [k][p] */
+	double y[FD_BK * FD_BQ];	/* [k][q] */
+};
+
+template <int FD_BK, int FD_NST, int MINW>
+__global__ void __launch_bounds__(GEMM_WG, MINW)
+k_gemm_f64_dma(const gemm_tile *__restrict__ tiles,
+    const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
+    const gemm_scalars *__restrict__ scp)
+{
+	constexpr int MT = 2, NT = 4;
+	constexpr int NXD = FD_BK / 2, NYD = FD_BK;	/* X / Y DMAs per step */
+	constexpr int DPW = (NXD + NYD) / 4;		/* per wave */
+	static_assert(NXD % 4 == 0 || NXD == 4, "X DMAs per wave");
+	constexpr int WAIT = (DPW * (FD_NST - 2)) | (0x7 << 4) | (0xf << 8);
+	typedef fd_stage<FD_BK> stage_t;
+	__shared__ stage_t st[FD_NST];
+
+	const gemm_tile tl = tiles[blockIdx.x];
+	if (tl.cblock == 0xffffffffu)
+		return;
+	const gemm_cblock cb = cbs[tl.cblock];
+	const gemm_scalars sc = *scp;
+	const int tid = threadIdx.x, lane = tid & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+	const int wp = (wave & 1) * 32, wq = (wave >> 1) * 64;
+	const bool live = tl.p0 + wp < cb.p && tl.q0 + wq < cb.q;
+	const uint32_t pend = cb.pair_end;
+	const uint32_t pbeg = (sc.flags & 1) ? pend : cb.pair_begin;
+
+	d4 acc[MT][NT];
+#pragma unroll
+	for (int i = 0; i < MT; i++)
+#pragma unroll
+		for (int j = 0; j < NT; j++)
+			acc[i][j] = d4{0, 0, 0, 0};
+
+	/* step at column k0 of pair px into stage nx: wave w issues X DMAs
+	 * w, w + 4, ... (two columns each) and Y DMAs w, w + 4, ... */
+	auto fetch = [&](stage_t *__restrict__ nx, const gemm_pair px,
+	    uint32_t k0) {
+		const double *X = (const double *)(uintptr_t)px.x;
+		const double *Y = (const double *)(uintptr_t)px.y;
+		const uint32_t r2 = 2 * (lane & 31);
+#pragma unroll
+		for (int d = 0; d < NXD / 4; d++) {
+			const int xd = wave + 4 * d;
+			const int col = 2 * xd + (lane >> 5);
+			const uint32_t k = k0 + col, p = tl.p0 + r2;
+			const void *src = &fd_zero_page[lane];
+			if (k < px.k && p < cb.p)
+				src = X + (size_t)p + (size_t)k * px.ldx;
+			__builtin_amdgcn_global_load_lds((fd_gbl_t *)src,
+			    (fd_lds_t *)&nx->x[2 * xd * FD_BP], 16, 0, 0);
+		}
+#pragma unroll
+		for (int d = 0; d < NYD / 4; d++) {
+			const int col = wave + 4 * d;
+			const uint32_t k = k0 + col, q = tl.q0 + 2 * lane;
+			const void *src = &fd_zero_page[lane];
+			if (k < px.k && q < cb.q)
+				src = Y + (size_t)q + (size_t)k * px.ldy;
+			__builtin_amdgcn_global_load_lds((fd_gbl_t *)src,
+			    (fd_lds_t *)&nx->y[col * FD_BQ], 16, 0, 0);
+		}
+	};
+	auto advance = [&](uint32_t &i, uint32_t &k, gemm_pair &p) {
+		k += FD_BK;
+		if (k >= p.k) {
+			k = 0;
+			if (++i < pend)
+				p = pairs[i];
+		}
+	};
+
+	/* compute cursor (ip, kc, pr); fetch cursor FD_NST - 1 steps ahead */
+	uint32_t ip = pbeg, kc = 0, fip = pbeg, fk = 0;
+	gemm_pair pr, fpr;
+	if (ip < pend) {
+		pr = pairs[ip];
+		fpr = pr;
+	}
+#pragma unroll
+	for (int s = 0; s < FD_NST - 1; s++)
+		if (fip < pend) {
+			fetch(&st[s], fpr, fk);
+			advance(fip, fk, fpr);
+		}
+	__builtin_amdgcn_s_waitcnt(0);
+	__syncthreads();
+	auto step = [&](const stage_t *__restrict__ cur,
+	    stage_t *__restrict__ nxt) {
+		const bool more = fip < pend;
+		if (more) {
+			fetch(nxt, fpr, fk);
+			advance(fip, fk, fpr);
+		}
+		if (live) {
+			const double al = __dmul_rn(sc.alpha_re, pr.a_re);
+#pragma unroll
+			for (int kk = 0; kk < FD_BK; kk += 4) {
+				const int kr = kk + (lane >> 4), c16 = lane & 15;
+				double xa[MT], yb[NT];
+#pragma unroll
+				for (int i = 0; i < MT; i++)
+					xa[i] = cur->x[kr * FD_BP + wp + i * 16 + c16] * al;
+#pragma unroll
+				for (int j = 0; j < NT; j++)
+					yb[j] = cur->y[kr * FD_BQ + wq + j * 16 + c16];
+#pragma unroll
+				for (int i = 0; i < MT; i++)
+#pragma unroll
+					for (int j = 0; j < NT; j++)
+						acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+						    yb[j], xa[i], acc[i][j], 0, 0, 0);
+			}
+		}
+		/* this step's stage is read (every LDS value is used by now) and
+		 * this wave's DMAs of the next step have landed */
+		if (more)
+			__builtin_amdgcn_s_waitcnt(WAIT);
+		else
+			__builtin_amdgcn_s_waitcnt(0);
+		__atomic_signal_fence(__ATOMIC_SEQ_CST);
+		__builtin_amdgcn_s_barrier();
+		__atomic_signal_fence(__ATOMIC_SEQ_CST);
+		advance(ip, kc, pr);
+	};
+	int cs = 0;
+	while (ip < pend) {
+		const int ns = cs + FD_NST - 1 < FD_NST ? cs + FD_NST - 1 : cs - 1;
+		step(&st[cs], &st[ns]);
+		cs = cs + 1 < FD_NST ? cs + 1 : 0;
+	}
+
+	/* D[q][p] of v_mfma_f64_16x16x4_f64: lane -> p = lane % 16,
+	 * register r -> q = 4 * r + lane / 16 */
+	double *C = (double *)(uintptr_t)cb.c;
+	const bool bz = sc.flags & 2;
+	const bool none = pbeg == pend;
+#pragma unroll
+	for (int i = 0; i < MT; i++)
+#pragma unroll
+		for (int j = 0; j < NT; j++)
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				uint32_t p = tl.p0 + wp + i * 16 + (lane & 15);
+				uint32_t q = tl.q0 + wq + j * 16 + 4 * r + (lane >> 4);
+				if (p < cb.p && q < cb.q) {
+					size_t o = (size_t)p + (size_t)q * cb.ldc;
+					double v;
+					if (bz)
+						v = none ? 0.0 : acc[i][j][r];
+					else if (none)
+						v = __dmul_rn(sc.beta_re, C[o]);
+					else
+						v = __dadd_rn(__dmul_rn(sc.beta_re, C[o]),
+						    acc[i][j][r]);
+					C[o] = v;
+				}
+			}
+}
+
 /* ---- FP32 / complex / reference kernel ---------------------------- */
 
 template <typename R> struct cx { R re, im; };
@@ -439,9 +633,35 @@ order_tiles(const contract_plan *pl, uint32_t BP, uint32_t BQ,
 			tiles[s * NXCD + x] = bins[x][s];
 }
 
+/* Every X and Y panel P-contiguous, rows and leading dimension even, base
+ * 16-byte aligned: what k_gemm_f64_dma's 16-byte row-pair DMAs need.
+ * XM_GEMM_DMA=0 turns the DMA kernel off. */
+static bool
+f64_dma_eligible(const contract_plan *pl)
+{
+	const char *e = getenv("XM_GEMM_DMA");
+	if ((e && *e == '0') || pl->type != XM_SCALAR_DOUBLE ||
+	    pl->precision != XM_PRECISION_NATIVE || use_simple())
+		return false;
+	for (const gemm_cblock &cb : pl->cblocks) {
+		if (cb.pair_begin == cb.pair_end)
+			continue;
+		if (cb.p % 2 || cb.q % 2)
+			return false;
+		for (uint32_t i = cb.pair_begin; i < cb.pair_end; i++) {
+			const gemm_pair &g = pl->pairs[i];
+			if (g.xmode != PANEL_PCONTIG || g.ymode != PANEL_PCONTIG ||
+			    g.ldx % 2 || g.ldy % 2 || g.x % 16 || g.y % 16)
+				return false;
+		}
+	}
+	return true;
+}
+
 void
 build_tiles(contract_plan *pl)
 {
+	pl->f64_dma = f64_dma_eligible(pl);
 	std::vector<std::pair<uint32_t, uint32_t>> cand;
 	tile_candidates(pl, cand);
 	pl->ncand = (int)std::min<size_t>(cand.size(), MAX_TILE_CANDIDATES);
@@ -511,6 +731,9 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 	case XM_SCALAR_DOUBLE:
 		if (use_simple())
 			k_gemm_simple<double><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		else if (pl->cand_bq[pl->tile_active] == 128 && pl->f64_dma)
+			k_gemm_f64_dma<16, 2, 3><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		else if (pl->cand_bq[pl->tile_active] == 128)
 			k_gemm_f64<64, 128, 16><<<g, b, 0, s>>>(pl->d_tiles,
