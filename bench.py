@@ -337,6 +337,48 @@ def measured_traffic(kernel):
         return None
 
 
+def permute_mode(lib, args):
+    """xm_copy(b, 1, a, "ijkl", perm) on 4-D FP64 tensors: 128^4 split into
+    blocks of 32 and of 16, and 112^4 split into uneven blocks of 4-28 (the
+    ring's sizes); GB/s = (bytes read + written) / median wall time of the
+    call (warm: job lists cached), with the jobs' permute form."""
+    import random
+    from libxm_amd import xm
+    from libxm_amd.spec import Spec
+    rng = random.Random(args.seed + 3)
+    cuts, x = [], 0
+    while True:
+        x += rng.randint(4, 28)
+        if x >= 112 - 3:
+            break
+        cuts.append(x)
+    cases = (("128^4 blocks 32", 128, [k * 32 for k in range(1, 4)]),
+             ("128^4 blocks 16", 128, [k * 16 for k in range(1, 8)]),
+             ("112^4 blocks 4-28", 112, cuts))
+    out = {"mode": "permute", "unit": "GB/s", "peak": HBM_PEAK, "cases": []}
+    for name, n, pts in cases:
+        sp = Spec((n,) * 4, {d: pts for d in range(4)},
+                  xm.SCALAR_DOUBLE).all_canonical()
+        alloc = lib.allocator()
+        a, b = sp.build(lib, alloc), sp.build(lib, alloc)
+        lib.fill_random(a, 1)
+        nbytes = 2 * 8 * n ** 4
+        for perm in ("ijkl", "jikl", "ikjl", "lkji", "klij", "ljki"):
+            ms = timed(lib, lambda: lib.copy(b, 1.0, a, "ijkl", perm),
+                       max(args.steps, 5))
+            tiled, gather = lib.last_permute()
+            out["cases"].append({"tensor": name, "perm": perm,
+                                 "bytes": nbytes, "ms": ms,
+                                 "gbs": nbytes / ms / 1e6,
+                                 "frac": nbytes / ms / 1e6 / HBM_PEAK,
+                                 "jobs_tiled": tiled,
+                                 "jobs_gather": gather})
+        lib.tensor_free(a)
+        lib.tensor_free(b)
+        lib.allocator_destroy(alloc)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -349,13 +391,16 @@ def main():
     ap.add_argument("--slices", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true")
-    ap.add_argument("--mode", choices=["contract", "copy", "ring", "precision"],
+    ap.add_argument("--mode", choices=["contract", "copy", "ring", "precision",
+                                       "permute"],
                     default="contract",
                     help="copy: xm_copy of a 4 GiB FP64 tensor, a plain "
                          "copy of known size (HBM counter calibration); "
                          "ring: the o=96 v=480 ring in FP64 and complex64 "
                          "with xm_add / xm_set; precision: the o=64 v=512 "
-                         "ladder over six decades in FP64, split and FP32")
+                         "ladder over six decades in FP64, split and FP32; "
+                         "permute: xm_copy under six permutations of 4-D "
+                         "tensors with blocks of 32, 16 and 4-28")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build runs on a single "
@@ -370,6 +415,8 @@ def main():
         return ring_mode(lib, args)
     if args.mode == "precision":
         return precision_mode(lib, args)
+    if args.mode == "permute":
+        return permute_mode(lib, args)
     w = workloads.ladder_d2h(args.o, args.v, args.seed)
     log("workload", json.dumps(w.describe()))
     tb = time.perf_counter()

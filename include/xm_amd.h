@@ -79,9 +79,17 @@ typedef struct {
 	 * shape was still being tuned (the call timed one candidate) */
 	int tile_p, tile_q, tile_tuning;
 	int graph;		/* 1 if the call replayed the plan's hipGraph */
+	/* permute jobs of the call (operand panels, C in, C out) moved by
+	 * the LDS-tiled kernel, and by the gather kernel */
+	size_t permute_tiled, permute_gather;
 } xm_contract_stats_t;
 
 void xm_contract_get_stats(xm_contract_stats_t *stats);
+
+/* Jobs (one per canonical block written) of the last xm_add, xm_copy,
+ * xm_mul or xm_div moved by the LDS-tiled permute kernel and by the
+ * gather kernel.  No libxm counterpart. */
+void xm_amd_last_permute(size_t *tiled, size_t *gather);
 
 /*
  * The block pairs xm_contract multiplies for each canonical block of c, in

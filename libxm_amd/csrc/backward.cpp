@@ -86,6 +86,8 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 	acc->panel_bytes += st.panel_bytes;
 	acc->cin_bytes += st.cin_bytes;
 	acc->cout_bytes += st.cout_bytes;
+	acc->permute_tiled += st.permute_tiled;
+	acc->permute_gather += st.permute_gather;
 	acc->precision = st.precision;
 	acc->nslices = st.nslices;
 	(void)x_first;
@@ -127,12 +129,7 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 	}
 	hipStream_t s = xm_stream();
 	for (auto &wv : waves) {
-		size_t c = 0;
-		for (auto &j : wv) {
-			j.first_chunk = c;
-			c += (j.n + 4095) / 4096;
-		}
-		launch_elem(wv.data(), wv.size(), c, ELEM_ACCUM, x->type, x->type,
+		launch_elem(wv.data(), wv.size(), ELEM_ACCUM, x->type, x->type,
 		    0, s);
 		HIP_OK(FN, hipStreamSynchronize(s));
 	}

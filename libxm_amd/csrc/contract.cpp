@@ -388,17 +388,6 @@ struct plan_builder {
 	}
 };
 
-static void
-finish_chunks(std::vector<elem_job> &jobs, size_t *nchunks)
-{
-	size_t c = 0;
-	for (auto &j : jobs) {
-		j.first_chunk = c;
-		c += (j.n + 4095) / 4096;
-	}
-	*nchunks = c;
-}
-
 static contract_plan *
 build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
     const masks_t &mk, int precision)
@@ -572,9 +561,6 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 	}
 	free(list);
 	pl->nunfold = pl->unfold_jobs.size();
-	finish_chunks(pl->unfold_jobs, &pl->unfold_chunks);
-	finish_chunks(pl->cin_jobs, &pl->cin_chunks);
-	finish_chunks(pl->cout_jobs, &pl->cout_chunks);
 	return pl;
 }
 
@@ -652,9 +638,10 @@ plan_upload(contract_plan *pl)
 	for (int c = 0; c < pl->ncand; c++)
 		add(pl->cand_tiles[c].data(),
 		    pl->cand_tiles[c].size() * sizeof(gemm_tile));
-	std::vector<char> ju = elem_jobs_pack(pl->unfold_jobs);
-	std::vector<char> ji = elem_jobs_pack(pl->cin_jobs);
-	std::vector<char> jo = elem_jobs_pack(pl->cout_jobs);
+	const size_t esz = elem_size(pl->type);
+	std::vector<char> ju = elem_jobs_pack(pl->unfold_jobs, esz, &pl->unfold_l);
+	std::vector<char> ji = elem_jobs_pack(pl->cin_jobs, esz, &pl->cin_l);
+	std::vector<char> jo = elem_jobs_pack(pl->cout_jobs, esz, &pl->cout_l);
 	add(ju.data(), ju.size());
 	add(ji.data(), ji.size());
 	add(jo.data(), jo.size());
@@ -826,12 +813,12 @@ launch_sequence(contract_plan *pl, uint32_t flags, hipStream_t s, bool stamps)
 			HIP_OK(fn, hipEventRecord(pl->ev[i], s));
 	};
 	mark(0);
-	if (do_pairs && pl->unfold_chunks)
-		launch_elem_packed(pl->d_unfold, pl->unfold_jobs.size(),
-		    pl->unfold_chunks, ELEM_GATHER, pl->type, pl->type, 0, s);
-	if (!(flags & 2) && pl->cin_chunks)
-		launch_elem_packed(pl->d_cin, pl->cin_jobs.size(),
-		    pl->cin_chunks, ELEM_GATHER, pl->type, pl->type, 0, s);
+	if (do_pairs)
+		launch_elem_packed(pl->d_unfold, pl->unfold_l, ELEM_GATHER,
+		    pl->type, pl->type, 0, s);
+	if (!(flags & 2))
+		launch_elem_packed(pl->d_cin, pl->cin_l, ELEM_GATHER,
+		    pl->type, pl->type, 0, s);
 	if (do_pairs && pl->precision == XM_PRECISION_SPLIT_BF16)
 		split_slices(pl, s);
 	mark(1);
@@ -840,9 +827,8 @@ launch_sequence(contract_plan *pl, uint32_t flags, hipStream_t s, bool stamps)
 	else
 		launch_gemm(pl, s);
 	mark(2);
-	if (pl->cout_chunks)
-		launch_elem_packed(pl->d_cout, pl->cout_jobs.size(),
-		    pl->cout_chunks, ELEM_GATHER, pl->type, pl->type, 0, s);
+	launch_elem_packed(pl->d_cout, pl->cout_l, ELEM_GATHER, pl->type,
+	    pl->type, 0, s);
 	mark(3);
 }
 
@@ -889,6 +875,11 @@ xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
 	st.tile_q = (int)pl->cand_bq[pl->tile_active];
 	st.tile_tuning = pl->tile_choice < 0;
 	st.graph = pl->last_graph;
+	const bool ops = alpha != xm_scalar_t(0), cin = beta != xm_scalar_t(0);
+	st.permute_tiled = (ops ? pl->unfold_l.ntiled : 0) +
+	    (cin ? pl->cin_l.ntiled : 0) + pl->cout_l.ntiled;
+	st.permute_gather = (ops ? pl->unfold_l.ngather : 0) +
+	    (cin ? pl->cin_l.ngather : 0) + pl->cout_l.ngather;
 	gemm_tile_record(pl, (float)st.gemm_ms);
 }
 

@@ -69,7 +69,7 @@ struct contract_plan {
 
 	/* permute jobs: operand panels, C in, C out */
 	std::vector<elem_job> unfold_jobs, cin_jobs, cout_jobs;
-	size_t unfold_chunks = 0, cin_chunks = 0, cout_chunks = 0;
+	elem_launch unfold_l, cin_l, cout_l;	/* set by the upload */
 
 	/* device memory */
 	size_t ws_bytes = 0;

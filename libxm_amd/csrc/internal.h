@@ -161,9 +161,18 @@ enum elem_op {
 	ELEM_ACCUM = 5,		/* dst = dst + f*src               (adjoint fold) */
 };
 
-void launch_elem(const elem_job *djobs, size_t njobs, size_t nchunks,
-    int op, xm_scalar_type_t dtype, xm_scalar_type_t stype,
-    xm_scalar_t alpha, hipStream_t s);
+/* A packed job list: gather-form records first, tiled-form records at
+ * byte offset toff; jobs and workgroups of each form. */
+struct elem_launch {
+	size_t ngather = 0, gchunks = 0;
+	size_t ntiled = 0, tchunks = 0;
+	size_t toff = 0;
+};
+
+elem_launch launch_elem(const elem_job *jobs, size_t njobs, int op,
+    xm_scalar_type_t dtype, xm_scalar_type_t stype, xm_scalar_t alpha,
+    hipStream_t s);
+bool elem_tiling_enabled(void);
 
 /* Per-block dot products reduced in a fixed order (xm_dot). */
 void launch_dot(const elem_job *djobs, size_t njobs, size_t nchunks,
@@ -207,9 +216,10 @@ struct gemm_scalars {
 
 /* Pack element jobs into their device form; launch from packed jobs
  * already in device memory. */
-std::vector<char> elem_jobs_pack(const std::vector<elem_job> &jobs);
-void launch_elem_packed(const void *djobs, size_t njobs, size_t nchunks,
-    int op, xm_scalar_type_t dtype, xm_scalar_type_t stype,
-    xm_scalar_t alpha, hipStream_t s);
+std::vector<char> elem_jobs_pack(const std::vector<elem_job> &jobs,
+    size_t src_esz, elem_launch *l);
+void launch_elem_packed(const void *djobs, const elem_launch &l, int op,
+    xm_scalar_type_t dtype, xm_scalar_type_t stype, xm_scalar_t alpha,
+    hipStream_t s);
 
 #endif /* XM_INTERNAL_H */

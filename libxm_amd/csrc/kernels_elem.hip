@@ -204,14 +204,63 @@ fdiv_q(uint32_t n, const fastdiv &f)
 	return (t + ((n - t) >> f.s1)) >> f.s2;
 }
 
-/* Device form of one job: extents as fast divisors. */
+/*
+ * Device forms of one job.  The destination block is walked in its own
+ * (canonical) order; the host merges dims the source walks as one run.
+ *
+ * Gather form (elem_job_dev, kernel k_elem): workgroup w of the job maps
+ * destination elements [w*CHUNK, (w+1)*CHUNK); writes are contiguous and
+ * reads follow the source strides.  Used where the source's fastest dim
+ * is destination dim 0, where dims 0..t (t the source's fastest) span at
+ * most one chunk, so each source cache line is used up inside one
+ * workgroup, and for absent sources.
+ *
+ * Tiled form (elem_tile_dev, kernel k_permute): destination dim 0 (extent
+ * A) against the source's fastest dim t (extent B).  A workgroup moves NS
+ * slabs of one TA x TB tile, the slabs being consecutive values of the
+ * remaining ("batch") dims: it reads them in source order (runs of TB
+ * along dim t) into LDS, then writes them in destination order (runs of TA
+ * along dim 0).  Without it, the lines of a far-transposed source are
+ * fetched once per element by workgroups spread over the XCDs' L2s.
+ */
+#define TILE_MAX 32			/* TA, TB <= 32 */
+#define TILE_BYTES (64 * 33 * 8)	/* tile storage, rows padded to TB + 1 */
+#define TILE_NS_MAX 128
+#define TILE_LDS (TILE_BYTES + 2 * TILE_NS_MAX * 8)
+
 struct elem_job_dev {
 	uint64_t dst, src, n, first_chunk;
 	uint32_t ndim, src_zero;
-	fastdiv div[XM_ELEM_MAXDIM];
-	uint64_t sstride[XM_ELEM_MAXDIM];
+	fastdiv div[XM_ELEM_MAXDIM];	/* merged destination extents */
+	uint64_t sstride[XM_ELEM_MAXDIM];	/* and their source strides */
 	double f_re, f_im;
 };
+
+struct elem_tile_dev {
+	uint64_t dst, src, first_chunk;
+	double f_re, f_im;
+	uint32_t nbd;			/* batch dims */
+	uint32_t A, B, TA, TB, NS, nta, ntb;
+	uint64_t nbatch;
+	uint64_t sa, sb, db;		/* source strides of dims 0 and t;
+					 * destination stride of t */
+	fastdiv f_tatb, f_tb, f_ta;
+	fastdiv bdiv[XM_ELEM_MAXDIM];	/* batch extents, destination order */
+	uint64_t bs[XM_ELEM_MAXDIM], bd[XM_ELEM_MAXDIM];
+};
+
+/* Job record -> LDS, one dword per thread. */
+template <typename J>
+__device__ __forceinline__ void
+load_job(J *dst, const J *src)
+{
+	constexpr uint32_t W = sizeof(J) / 4;
+	static_assert(W <= WG && sizeof(J) % 4 == 0, "job record");
+	if (threadIdx.x < W)
+		((uint32_t *)dst)[threadIdx.x] =
+		    ((const uint32_t *)src)[threadIdx.x];
+	__syncthreads();
+}
 
 __device__ __forceinline__ uint64_t
 src_offset(const elem_job_dev &jb, uint32_t i)
@@ -226,52 +275,134 @@ src_offset(const elem_job_dev &jb, uint32_t i)
 	return off;
 }
 
+template <int OP> struct reads_dst {
+	static constexpr bool v = OP == ELEM_AXPY || OP == ELEM_MUL ||
+	    OP == ELEM_DIV || OP == ELEM_ACCUM;
+};
+
+/* The operation on one destination element: x is the source element (0
+ * when the source is absent), d the old destination value. */
+template <typename D, typename S, int OP>
+__device__ __forceinline__ D
+elem_op(D d, S x, bool present, S f, D alpha, int alpha_zero)
+{
+	if (OP == ELEM_GATHER) {
+		return conv<D>(x);
+	} else if (OP == ELEM_COPY_SCALED) {
+		/* libxm src/xm.c:131-141: scale in b's type, convert */
+		return present ? conv<D>(emul(x, f)) : mk<D>(0.0, 0.0);
+	} else if (OP == ELEM_AXPY) {
+		/* libxm src/xm.c:218-239 with src/scalar.c:193-239 */
+		D tmp = present ? conv<D>(emul(x, f)) : mk<D>(0.0, 0.0);
+		if (alpha_zero)
+			return tmp;
+		return eadd(emul(alpha, d), emul(mk<D>(1.0, 0.0), tmp));
+	} else if (OP == ELEM_MUL) {
+		/* libxm src/xm.c:306-318 with src/scalar.c:241-283 */
+		return present ? emul(emul(conv<D>(f), d), conv<D>(x)) :
+		    mk<D>(0.0, 0.0);
+	} else if (OP == ELEM_DIV) {
+		/* libxm src/xm.c:389-399 with src/scalar.c:285-327 */
+		return ediv(d, emul(conv<D>(f), conv<D>(x)));
+	} else {	/* ELEM_ACCUM */
+		return present ? eadd(d, conv<D>(emul(x, f))) : d;
+	}
+}
+
 template <typename D, typename S, int OP>
 __global__ void __launch_bounds__(WG)
 k_elem(const elem_job_dev *jobs, uint32_t njobs, double al_re, double al_im,
     int alpha_zero)
 {
-	uint64_t chunk = blockIdx.x;
-	uint32_t j = find_job(jobs, njobs, chunk);
+	const uint64_t chunk = blockIdx.x;
+	const uint32_t j = find_job(jobs, njobs, chunk);
 	__shared__ elem_job_dev jb;
-	if (threadIdx.x == 0)
-		jb = jobs[j];
-	__syncthreads();
-	uint64_t base = (chunk - jb.first_chunk) * CHUNK;
+	load_job(&jb, jobs + j);
+	const uint64_t base = (chunk - jb.first_chunk) * CHUNK;
 	D *dst = (D *)(uintptr_t)jb.dst;
 	const S *src = (const S *)(uintptr_t)jb.src;
 	const S f = mk<S>(jb.f_re, jb.f_im);
 	const D alpha = mk<D>(al_re, al_im);
+	const bool present = !jb.src_zero;
 	for (uint64_t i = base + threadIdx.x; i < jb.n && i < base + CHUNK;
 	    i += WG) {
 		S x = mk<S>(0.0, 0.0);
-		if (!jb.src_zero)
+		if (present)
 			x = src[src_offset(jb, (uint32_t)i)];
-		if (OP == ELEM_GATHER) {
-			dst[i] = conv<D>(x);
-		} else if (OP == ELEM_COPY_SCALED) {
-			/* libxm src/xm.c:131-141: scale in b's type, convert */
-			dst[i] = jb.src_zero ? mk<D>(0.0, 0.0) :
-			    conv<D>(emul(x, f));
-		} else if (OP == ELEM_AXPY) {
-			/* libxm src/xm.c:218-239 with src/scalar.c:193-239 */
-			D tmp = jb.src_zero ? mk<D>(0.0, 0.0) :
-			    conv<D>(emul(x, f));
-			if (alpha_zero)
-				dst[i] = tmp;
-			else
-				dst[i] = eadd(emul(alpha, dst[i]),
-				    emul(mk<D>(1.0, 0.0), tmp));
-		} else if (OP == ELEM_MUL) {
-			/* libxm src/xm.c:306-318 with src/scalar.c:241-283 */
-			dst[i] = jb.src_zero ? mk<D>(0.0, 0.0) :
-			    emul(emul(conv<D>(f), dst[i]), conv<D>(x));
-		} else if (OP == ELEM_DIV) {
-			/* libxm src/xm.c:389-399 with src/scalar.c:285-327 */
-			dst[i] = ediv(dst[i], emul(conv<D>(f), conv<D>(x)));
-		} else if (OP == ELEM_ACCUM) {
-			if (!jb.src_zero)
-				dst[i] = eadd(dst[i], conv<D>(emul(x, f)));
+		D d = mk<D>(0.0, 0.0);
+		if (reads_dst<OP>::v)
+			d = dst[i];
+		dst[i] = elem_op<D, S, OP>(d, x, present, f, alpha, alpha_zero);
+	}
+}
+
+template <typename D, typename S, int OP>
+__global__ void __launch_bounds__(WG)
+k_permute(const elem_tile_dev *jobs, uint32_t njobs, double al_re,
+    double al_im, int alpha_zero)
+{
+	extern __shared__ __align__(16) char tile_lds[];
+	S *tile = (S *)tile_lds;
+	uint64_t *sbase = (uint64_t *)(tile_lds + TILE_BYTES);
+	uint64_t *dbase = sbase + TILE_NS_MAX;
+	const uint64_t chunk = blockIdx.x;
+	const uint32_t j = find_job(jobs, njobs, chunk);
+	__shared__ elem_tile_dev jb;
+	load_job(&jb, jobs + j);
+	const uint64_t local = chunk - jb.first_chunk;
+	const uint32_t ia = (uint32_t)(local % jb.nta);
+	const uint64_t r1 = local / jb.nta;
+	const uint32_t ib = (uint32_t)(r1 % jb.ntb);
+	const uint64_t s0 = r1 / jb.ntb * jb.NS;
+	const uint32_t a0 = ia * jb.TA, b0 = ib * jb.TB;
+	const uint32_t ta = min(jb.TA, jb.A - a0), tb = min(jb.TB, jb.B - b0);
+	const uint32_t ns = (uint32_t)min((uint64_t)jb.NS, jb.nbatch - s0);
+	for (uint32_t s = threadIdx.x; s < ns; s += WG) {
+		uint32_t q = (uint32_t)(s0 + s);
+		uint64_t so = a0 * jb.sa + b0 * jb.sb, dof = a0 + b0 * jb.db;
+		for (uint32_t k = 0; k < jb.nbd; k++) {
+			const uint32_t qq = fdiv_q(q, jb.bdiv[k]);
+			const uint32_t r = q - qq * jb.bdiv[k].d;
+			so += r * jb.bs[k];
+			dof += r * jb.bd[k];
+			q = qq;
+		}
+		sbase[s] = so;
+		dbase[s] = dof;
+	}
+	__syncthreads();
+	D *dst = (D *)(uintptr_t)jb.dst;
+	const S *src = (const S *)(uintptr_t)jb.src;
+	const S f = mk<S>(jb.f_re, jb.f_im);
+	const D alpha = mk<D>(al_re, al_im);
+	const uint32_t TA = jb.TA, TB = jb.TB, TATB = TA * TB;
+	const uint32_t tot = jb.NS * TATB, row = TB + 1;
+	/* source order (b fastest) into the tile */
+	for (uint32_t e = threadIdx.x; e < tot; e += WG) {
+		const uint32_t s = fdiv_q(e, jb.f_tatb);
+		const uint32_t r = e - s * TATB;
+		const uint32_t a = fdiv_q(r, jb.f_tb);
+		const uint32_t b = r - a * TB;
+		S v = mk<S>(0.0, 0.0);
+		if (s < ns && a < ta && b < tb)
+			v = src[sbase[s] + a * jb.sa + b * jb.sb];
+		tile[(s * TA + a) * row + b] = v;
+	}
+	__syncthreads();
+	/* destination order (a fastest) out of the tile */
+	for (uint32_t e = threadIdx.x; e < tot; e += WG) {
+		const uint32_t s = fdiv_q(e, jb.f_tatb);
+		const uint32_t r = e - s * TATB;
+		const uint32_t b = fdiv_q(r, jb.f_ta);
+		const uint32_t a = r - b * TA;
+		if (s < ns && a < ta && b < tb) {
+			const uint64_t i = dbase[s] + a + b * jb.db;
+			D d = mk<D>(0.0, 0.0);
+			if (reads_dst<OP>::v)
+				d = dst[i];
+			dst[i] = elem_op<D, S, OP>(d,
+			    tile[(s * TA + a) * row + b], true, f, alpha,
+			    alpha_zero);
 		}
 	}
 }
@@ -297,9 +428,7 @@ k_dot(const elem_job_dev *jobs, uint32_t njobs, double *partials)
 	uint32_t j = find_job(jobs, njobs, chunk);
 	__shared__ elem_job_dev jb;
 	__shared__ double red[2][WG];
-	if (threadIdx.x == 0)
-		jb = jobs[j];
-	__syncthreads();
+	load_job(&jb, jobs + j);
 	uint64_t base = (chunk - jb.first_chunk) * CHUNK;
 	const S *a = (const S *)(uintptr_t)jb.dst;	/* a, canonical order */
 	const S *b = (const S *)(uintptr_t)jb.src;	/* b, permuted */
@@ -393,18 +522,79 @@ launch_fill_random(const seg_desc *dsegs, size_t nseg, size_t nchunks,
 	check_launch("xm_amd_fill_random");
 }
 
-static std::vector<elem_job_dev>
-to_dev(const elem_job *jobs, size_t n)
+/*
+ * Tiled form of a job (dims/str: merged destination extents and their
+ * source strides), or false where the gather form reads whole cache lines
+ * anyway (see elem_tile_dev).
+ */
+static bool
+plan_tiled(elem_tile_dev &b, const uint32_t *dims, const uint64_t *str,
+    uint32_t nd, size_t esz)
 {
-	std::vector<elem_job_dev> out(n);
+	uint32_t t = 0;
+	for (uint32_t d = 1; d < nd; d++)
+		if (str[d] < str[t])
+			t = d;
+	if (t == 0)
+		return false;
+	uint64_t span = 1;
+	for (uint32_t d = 0; d <= t; d++)
+		span *= dims[d];
+	if (span <= CHUNK)
+		return false;
+	uint64_t dstr[XM_ELEM_MAXDIM], s = 1;
+	for (uint32_t d = 0; d < nd; d++) {
+		dstr[d] = s;
+		s *= dims[d];
+	}
+	b.A = dims[0];
+	b.B = dims[t];
+	b.nta = (b.A + TILE_MAX - 1) / TILE_MAX;
+	b.TA = (b.A + b.nta - 1) / b.nta;
+	b.ntb = (b.B + TILE_MAX - 1) / TILE_MAX;
+	b.TB = (b.B + b.ntb - 1) / b.ntb;
+	b.sa = str[0];
+	b.sb = str[t];
+	b.db = dstr[t];
+	b.nbd = 0;
+	b.nbatch = 1;
+	for (uint32_t d = 1; d < nd; d++) {
+		if (d == t)
+			continue;
+		b.bdiv[b.nbd] = make_fdiv(dims[d]);
+		b.bs[b.nbd] = str[d];
+		b.bd[b.nbd] = dstr[d];
+		b.nbd++;
+		b.nbatch *= dims[d];
+	}
+	for (uint32_t k = b.nbd; k < XM_ELEM_MAXDIM; k++)
+		b.bdiv[k] = make_fdiv(1);
+	uint64_t ns = (TILE_BYTES / esz) / ((uint64_t)b.TA * (b.TB + 1));
+	ns = std::min<uint64_t>(std::max<uint64_t>(ns, 1), TILE_NS_MAX);
+	b.NS = (uint32_t)std::min<uint64_t>(ns, b.nbatch);
+	b.f_tatb = make_fdiv(b.TA * b.TB);
+	b.f_tb = make_fdiv(b.TB);
+	b.f_ta = make_fdiv(b.TA);
+	return true;
+}
+
+/*
+ * Device records of a job list: gather records, then (at l->toff bytes)
+ * tiled records, each list with its own workgroup prefix.  esz: source
+ * element size (tile storage); tiling false keeps every job a gather.
+ */
+static std::vector<char>
+to_dev(const elem_job *jobs, size_t n, size_t esz, bool tiling,
+    elem_launch *l)
+{
+	std::vector<elem_job_dev> g;
+	std::vector<elem_tile_dev> t;
+	g.reserve(n);
+	uint64_t cg = 0, ct = 0;
 	for (size_t i = 0; i < n; i++) {
 		const elem_job &a = jobs[i];
-		elem_job_dev &b = out[i];
-		b.dst = a.dst;
-		b.src = a.src;
-		b.n = a.n;
-		b.first_chunk = a.first_chunk;
-		b.src_zero = a.src_zero;
+		if (a.n > 0xffffffffull)
+			XM_FATAL("libxm", "block of more than 2^32 elements");
 		/* merge dimension d + 1 into d where the source walks them as
 		 * one run (stride[d+1] = stride[d] * dims[d]): an identity
 		 * permutation becomes one dimension, one multiply per element */
@@ -422,6 +612,26 @@ to_dev(const elem_job *jobs, size_t n)
 			dims[nd] = a.dims[d];
 			str[nd++] = a.sstride[d];
 		}
+		elem_tile_dev tb;
+		memset(&tb, 0, sizeof tb);
+		if (tiling && !a.src_zero && nd >= 2 &&
+		    plan_tiled(tb, dims, str, nd, esz)) {
+			tb.dst = a.dst;
+			tb.src = a.src;
+			tb.f_re = a.f_re;
+			tb.f_im = a.f_im;
+			tb.first_chunk = ct;
+			ct += (uint64_t)tb.nta * tb.ntb *
+			    ((tb.nbatch + tb.NS - 1) / tb.NS);
+			t.push_back(tb);
+			continue;
+		}
+		elem_job_dev b;
+		memset(&b, 0, sizeof b);
+		b.dst = a.dst;
+		b.src = a.src;
+		b.n = a.n;
+		b.src_zero = a.src_zero;
 		b.ndim = nd;
 		for (uint32_t d = 0; d < XM_ELEM_MAXDIM; d++) {
 			b.div[d] = make_fdiv(d < nd && dims[d] ? dims[d] : 1);
@@ -429,62 +639,86 @@ to_dev(const elem_job *jobs, size_t n)
 		}
 		b.f_re = a.f_re;
 		b.f_im = a.f_im;
+		b.first_chunk = cg;
+		cg += (a.n + CHUNK - 1) / CHUNK;
+		g.push_back(b);
 	}
+	l->ngather = g.size();
+	l->gchunks = cg;
+	l->ntiled = t.size();
+	l->tchunks = ct;
+	l->toff = (g.size() * sizeof(elem_job_dev) + 255) / 256 * 256;
+	std::vector<char> out(l->toff + t.size() * sizeof(elem_tile_dev), 0);
+	if (!g.empty())
+		memcpy(out.data(), g.data(), g.size() * sizeof(elem_job_dev));
+	if (!t.empty())
+		memcpy(out.data() + l->toff, t.data(),
+		    t.size() * sizeof(elem_tile_dev));
 	return out;
+}
+
+template <typename D, typename S, int OP>
+static void
+elem_launch_op(const char *dj, const elem_launch &l, double ar, double ai,
+    int az, hipStream_t s)
+{
+	if (l.gchunks)
+		k_elem<D, S, OP><<<dim3((unsigned)l.gchunks), dim3(WG), 0, s>>>(
+		    (const elem_job_dev *)dj, (uint32_t)l.ngather, ar, ai, az);
+	if (l.tchunks)
+		k_permute<D, S, OP><<<dim3((unsigned)l.tchunks), dim3(WG),
+		    TILE_LDS, s>>>((const elem_tile_dev *)(dj + l.toff),
+		    (uint32_t)l.ntiled, ar, ai, az);
 }
 
 template <typename D, typename S>
 static void
-elem_dispatch(int op, const elem_job_dev *dj, uint32_t nj, size_t nchunks,
+elem_dispatch(int op, const char *dj, const elem_launch &l,
     xm_scalar_t alpha, hipStream_t s)
 {
-	dim3 g((unsigned)nchunks), b(WG);
 	double ar = alpha.real(), ai = alpha.imag();
 	int az = alpha == xm_scalar_t(0);
 	switch (op) {
 	case ELEM_COPY_SCALED:
-		k_elem<D, S, ELEM_COPY_SCALED><<<g, b, 0, s>>>(dj, nj, ar, ai, az);
+		elem_launch_op<D, S, ELEM_COPY_SCALED>(dj, l, ar, ai, az, s);
 		break;
 	case ELEM_AXPY:
-		k_elem<D, S, ELEM_AXPY><<<g, b, 0, s>>>(dj, nj, ar, ai, az);
+		elem_launch_op<D, S, ELEM_AXPY>(dj, l, ar, ai, az, s);
 		break;
 	case ELEM_MUL:
-		k_elem<D, S, ELEM_MUL><<<g, b, 0, s>>>(dj, nj, ar, ai, az);
+		elem_launch_op<D, S, ELEM_MUL>(dj, l, ar, ai, az, s);
 		break;
 	case ELEM_DIV:
-		k_elem<D, S, ELEM_DIV><<<g, b, 0, s>>>(dj, nj, ar, ai, az);
+		elem_launch_op<D, S, ELEM_DIV>(dj, l, ar, ai, az, s);
 		break;
 	case ELEM_GATHER:
-		k_elem<D, S, ELEM_GATHER><<<g, b, 0, s>>>(dj, nj, ar, ai, az);
+		elem_launch_op<D, S, ELEM_GATHER>(dj, l, ar, ai, az, s);
 		break;
 	case ELEM_ACCUM:
-		k_elem<D, S, ELEM_ACCUM><<<g, b, 0, s>>>(dj, nj, ar, ai, az);
+		elem_launch_op<D, S, ELEM_ACCUM>(dj, l, ar, ai, az, s);
 		break;
 	}
 }
 
 std::vector<char>
-elem_jobs_pack(const std::vector<elem_job> &jobs)
+elem_jobs_pack(const std::vector<elem_job> &jobs, size_t src_esz,
+    elem_launch *l)
 {
-	std::vector<elem_job_dev> hj = to_dev(jobs.data(), jobs.size());
-	std::vector<char> out(hj.size() * sizeof(elem_job_dev));
-	if (!out.empty())
-		memcpy(out.data(), hj.data(), out.size());
-	return out;
+	return to_dev(jobs.data(), jobs.size(), src_esz,
+	    elem_tiling_enabled(), l);
 }
 
 void
-launch_elem_packed(const void *djobs, size_t njobs, size_t nchunks, int op,
+launch_elem_packed(const void *djobs, const elem_launch &l, int op,
     xm_scalar_type_t dt, xm_scalar_type_t st, xm_scalar_t alpha,
     hipStream_t s)
 {
-	if (nchunks == 0 || njobs == 0)
+	if (l.gchunks == 0 && l.tchunks == 0)
 		return;
 	bump_data_epoch();
-	const elem_job_dev *dj = (const elem_job_dev *)djobs;
-	uint32_t nj = (uint32_t)njobs;
+	const char *dj = (const char *)djobs;
 #define PAIR(DT, ST, D, S) \
-	if (dt == DT && st == ST) { elem_dispatch<D, S>(op, dj, nj, nchunks, alpha, s); \
+	if (dt == DT && st == ST) { elem_dispatch<D, S>(op, dj, l, alpha, s); \
 		check_launch("libxm element kernel"); return; }
 	PAIR(XM_SCALAR_DOUBLE, XM_SCALAR_DOUBLE, double, double)
 	PAIR(XM_SCALAR_FLOAT, XM_SCALAR_FLOAT, float, float)
@@ -498,16 +732,31 @@ launch_elem_packed(const void *djobs, size_t njobs, size_t nchunks, int op,
 	XM_FATAL("libxm", "unsupported scalar conversion");
 }
 
-void
-launch_elem(const elem_job *jobs, size_t njobs, size_t nchunks, int op,
+elem_launch
+launch_elem(const elem_job *jobs, size_t njobs, int op,
     xm_scalar_type_t dt, xm_scalar_type_t st, xm_scalar_t alpha,
     hipStream_t s)
 {
-	if (nchunks == 0 || njobs == 0)
-		return;
-	std::vector<elem_job_dev> hj = to_dev(jobs, njobs);
-	void *dj = upload(1, hj.data(), hj.size() * sizeof(elem_job_dev));
-	launch_elem_packed(dj, njobs, nchunks, op, dt, st, alpha, s);
+	elem_launch l;
+	if (njobs == 0)
+		return l;
+	std::vector<char> hj = to_dev(jobs, njobs, elem_size(st),
+	    elem_tiling_enabled(), &l);
+	void *dj = upload(1, hj.data(), hj.size());
+	launch_elem_packed(dj, l, op, dt, st, alpha, s);
+	return l;
+}
+
+/* XM_ELEM_TILED=0 keeps every permute in the gather form. */
+bool
+elem_tiling_enabled(void)
+{
+	static int on = -1;
+	if (on < 0) {
+		const char *e = getenv("XM_ELEM_TILED");
+		on = !(e && *e == '0');
+	}
+	return on != 0;
 }
 
 /* The device wall clock (constant rate, hipDeviceAttributeWallClockRate
@@ -532,9 +781,11 @@ launch_dot(const elem_job *jobs, size_t njobs, size_t nchunks,
 {
 	if (nchunks == 0 || njobs == 0)
 		return;
-	std::vector<elem_job_dev> hj = to_dev(jobs, njobs);
-	elem_job_dev *dj = (elem_job_dev *)upload(1, hj.data(),
-	    hj.size() * sizeof(elem_job_dev));
+	elem_launch l;
+	std::vector<char> hj = to_dev(jobs, njobs, elem_size(type), false, &l);
+	if (l.gchunks != nchunks)
+		XM_FATAL("xm_dot", "chunk count mismatch");
+	elem_job_dev *dj = (elem_job_dev *)upload(1, hj.data(), hj.size());
 	dim3 g((unsigned)nchunks), b(WG);
 	switch (type) {
 	case XM_SCALAR_FLOAT: k_dot<float><<<g, b, 0, s>>>(dj, njobs, dpartials); break;

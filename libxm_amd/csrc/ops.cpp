@@ -6,7 +6,9 @@
 #include "contract_plan.h"
 
 #include <algorithm>
+#include <list>
 #include <map>
+#include <mutex>
 #include <set>
 
 /* Same checks and messages as libxm src/xm.c:169-189. */
@@ -47,6 +49,7 @@ block_strides(const xm_tensor_t *t, size_t flat, uint64_t *ls)
 		ls[k] = ss[b.perm.i[k]];
 }
 
+/* Workgroups of xm_dot's per-block partial sums (4096 elements each). */
 static size_t
 set_chunks(std::vector<elem_job> &jobs)
 {
@@ -141,15 +144,138 @@ stage_aliased(const xm_tensor_t *a, std::vector<elem_job> &jobs,
 	(void)a;
 }
 
+/*
+ * Cached job lists.  The jobs of an element operation depend only on the
+ * tensors' structures (block kinds, permutations, scalars and data
+ * addresses, all covered by tensor id and version), the index strings and
+ * the per-call factor that scales b's block scalars; a repeated call with
+ * the same key launches the device copy kept here, with no host work
+ * beyond validation.  Lists that read a block the same call writes are
+ * not cached: they read copies staged in scratch.
+ */
+struct op_key {
+	int op;
+	uint64_t ida, va, idb, vb;
+	std::string ia, ib;
+	double f_re, f_im;
+	bool operator==(const op_key &o) const {
+		return op == o.op && ida == o.ida && va == o.va &&
+		    idb == o.idb && vb == o.vb && ia == o.ia && ib == o.ib &&
+		    f_re == o.f_re && f_im == o.f_im;
+	}
+};
+
+struct op_entry {
+	void *dev;		/* packed jobs or segments, device copy */
+	elem_launch l;		/* xm_set: ngather segments, gchunks */
+};
+
+#define OP_CACHE_MAX 64
+static std::list<std::pair<op_key, op_entry>> op_cache;
+static std::mutex op_mu;
+
+static bool
+op_cache_find(const op_key &k, op_entry *e)
+{
+	std::lock_guard<std::mutex> g(op_mu);
+	for (auto it = op_cache.begin(); it != op_cache.end(); ++it)
+		if (it->first == k) {
+			op_cache.splice(op_cache.begin(), op_cache, it);
+			*e = it->second;
+			return true;
+		}
+	return false;
+}
+
+static op_entry
+op_cache_insert(const op_key &k, const void *host, size_t bytes,
+    const elem_launch &l)
+{
+	op_entry e{nullptr, l};
+	if (bytes) {
+		HIP_OK("libxm", hipMalloc(&e.dev, bytes));
+		HIP_OK("libxm", hipMemcpy(e.dev, host, bytes,
+		    hipMemcpyHostToDevice));
+	}
+	std::lock_guard<std::mutex> g(op_mu);
+	op_cache.push_front({k, e});
+	while (op_cache.size() > OP_CACHE_MAX) {
+		if (op_cache.back().second.dev)
+			HIP_OK("libxm", hipFree(op_cache.back().second.dev));
+		op_cache.pop_back();
+	}
+	return e;
+}
+
+/* Forget the job lists of tensor `id` (it is being freed). */
+void
+op_cache_drop(uint64_t id)
+{
+	std::lock_guard<std::mutex> g(op_mu);
+	for (auto it = op_cache.begin(); it != op_cache.end();) {
+		if (it->first.ida == id || it->first.idb == id) {
+			if (it->second.dev)
+				HIP_OK("libxm", hipFree(it->second.dev));
+			it = op_cache.erase(it);
+		} else {
+			++it;
+		}
+	}
+}
+
+static op_key
+make_key(int op, const xm_tensor_t *a, const xm_tensor_t *b,
+    const char *ia, const char *ib, xm_scalar_t f)
+{
+	return op_key{op, a->id, a->version, b ? b->id : 0,
+	    b ? b->version : 0, ia ? ia : "", ib ? ib : "", f.real(),
+	    f.imag()};
+}
+
+static elem_launch last_launch;
+
+/* Jobs of the last xm_add / xm_copy / xm_mul / xm_div in the LDS-tiled and
+ * in the gather form (no libxm counterpart). */
+XM_API void
+xm_amd_last_permute(size_t *tiled, size_t *gather)
+{
+	if (tiled)
+		*tiled = last_launch.ntiled;
+	if (gather)
+		*gather = last_launch.ngather;
+}
+
+/* Run the jobs `build` makes (or their cached device copy) as `op`. */
+template <typename B>
 static void
-run_jobs(const char *fn, std::vector<elem_job> &jobs, int op,
+run_jobs(const char *fn, const op_key &key, B build, int op,
     xm_scalar_type_t dt, xm_scalar_type_t st, xm_scalar_t alpha,
     const xm_tensor_t *a)
 {
 	hipStream_t s = xm_stream();
-	stage_aliased(a, jobs, elem_size(st), s);
-	size_t nchunks = set_chunks(jobs);
-	launch_elem(jobs.data(), jobs.size(), nchunks, op, dt, st, alpha, s);
+	op_entry e;
+	if (!op_cache_find(key, &e)) {
+		std::vector<elem_job> jobs = build();
+		std::set<uint64_t> dsts;
+		bool aliased = false;
+		for (auto &j : jobs)
+			dsts.insert(j.dst);
+		for (auto &j : jobs)
+			aliased |= !j.src_zero && dsts.count(j.src) != 0;
+		if (aliased) {
+			stage_aliased(a, jobs, elem_size(st), s);
+			last_launch = launch_elem(jobs.data(), jobs.size(), op,
+			    dt, st, alpha, s);
+			HIP_OK(fn, hipStreamSynchronize(s));
+			return;
+		}
+		elem_launch l;
+		std::vector<char> packed = elem_jobs_pack(jobs, elem_size(st),
+		    &l);
+		e = op_cache_insert(key, packed.data(), packed.size(), l);
+	}
+	last_launch = e.l;
+	launch_elem_packed(e.dev, e.l, op, dt, st, alpha, s);
 	HIP_OK(fn, hipStreamSynchronize(s));
 }
 
@@ -157,23 +283,32 @@ run_jobs(const char *fn, std::vector<elem_job> &jobs, int op,
 XM_API void
 xm_set(xm_tensor_t *a, xm_scalar_t x)
 {
-	std::vector<seg_desc> segs;
-	size_t c = 0;
-	for (size_t f = 0; f < a->blocks.size(); f++) {
-		if (a->blocks[f].type != XM_BLOCK_TYPE_CANONICAL)
-			continue;
-		xm_dim_t d = block_dims_flat(a, f);
-		seg_desc sd{a->blocks[f].data, xm_dim_dot(&d), c, f};
-		if (sd.n == 0)
-			continue;
-		c += (sd.n + 4095) / 4096;
-		segs.push_back(sd);
+	const op_key key = make_key(-1, a, nullptr, nullptr, nullptr, 0);
+	op_entry e;
+	if (!op_cache_find(key, &e)) {
+		std::vector<seg_desc> segs;
+		size_t c = 0;
+		for (size_t f = 0; f < a->blocks.size(); f++) {
+			if (a->blocks[f].type != XM_BLOCK_TYPE_CANONICAL)
+				continue;
+			xm_dim_t d = block_dims_flat(a, f);
+			seg_desc sd{a->blocks[f].data, xm_dim_dot(&d), c, f};
+			if (sd.n == 0)
+				continue;
+			c += (sd.n + 4095) / 4096;
+			segs.push_back(sd);
+		}
+		elem_launch l;
+		l.ngather = segs.size();
+		l.gchunks = c;
+		e = op_cache_insert(key, segs.data(),
+		    segs.size() * sizeof(seg_desc), l);
 	}
-	if (segs.empty())
+	if (e.l.ngather == 0)
 		return;
 	hipStream_t s = xm_stream();
-	void *d = upload(1, segs.data(), segs.size() * sizeof(seg_desc));
-	launch_fill_value((const seg_desc *)d, segs.size(), c, a->type, x, s);
+	launch_fill_value((const seg_desc *)e.dev, e.l.ngather, e.l.gchunks,
+	    a->type, x, s);
 	HIP_OK("xm_set", hipStreamSynchronize(s));
 }
 
@@ -210,14 +345,17 @@ xm_add(xm_scalar_t alpha, xm_tensor_t *a, xm_scalar_t beta,
 	xm_dim_t ca, cb;
 	pair_masks("xm_add", a, b, idxa, idxb, true, &ca, &cb);
 	xm_scalar_type_t t = a->type;
-	auto jobs = pair_jobs(a, b, ca, cb, [&](const xm_block &bb,
-	    xm_scalar_t *f) {
-		if (beta == xm_scalar_t(0))
-			return false;
-		*f = scalar_mul(beta, bb.scalar, t);
-		return true;
-	});
-	run_jobs("xm_add", jobs, ELEM_AXPY, t, t, alpha, a);
+	auto build = [&]() {
+		return pair_jobs(a, b, ca, cb, [&](const xm_block &bb,
+		    xm_scalar_t *f) {
+			if (beta == xm_scalar_t(0))
+				return false;
+			*f = scalar_mul(beta, bb.scalar, t);
+			return true;
+		});
+	};
+	run_jobs("xm_add", make_key(ELEM_AXPY, a, b, idxa, idxb, beta), build,
+	    ELEM_AXPY, t, t, alpha, a);
 }
 
 /* libxm src/xm.c:66-157: a = convert(s*s_b*perm(b)). */
@@ -230,14 +368,17 @@ xm_copy(xm_tensor_t *a, xm_scalar_t sc, const xm_tensor_t *b,
 	if (is_complex(a->type) != is_complex(b->type))
 		XM_FATAL("xm_copy", "unsupported scalar conversion");
 	xm_scalar_type_t tb = b->type;
-	auto jobs = pair_jobs(a, b, ca, cb, [&](const xm_block &bb,
-	    xm_scalar_t *f) {
-		if (sc == xm_scalar_t(0))
-			return false;
-		*f = scalar_mul(sc, bb.scalar, tb);
-		return true;
-	});
-	run_jobs("xm_copy", jobs, ELEM_COPY_SCALED, a->type, tb, 0, a);
+	auto build = [&]() {
+		return pair_jobs(a, b, ca, cb, [&](const xm_block &bb,
+		    xm_scalar_t *f) {
+			if (sc == xm_scalar_t(0))
+				return false;
+			*f = scalar_mul(sc, bb.scalar, tb);
+			return true;
+		});
+	};
+	run_jobs("xm_copy", make_key(ELEM_COPY_SCALED, a, b, idxa, idxb, sc),
+	    build, ELEM_COPY_SCALED, a->type, tb, 0, a);
 }
 
 /* libxm src/xm.c:251-332: a = (s_b*a)*perm(b). */
@@ -247,12 +388,15 @@ xm_mul(xm_tensor_t *a, const xm_tensor_t *b, const char *idxa,
 {
 	xm_dim_t ca, cb;
 	pair_masks("xm_mul", a, b, idxa, idxb, true, &ca, &cb);
-	auto jobs = pair_jobs(a, b, ca, cb, [&](const xm_block &bb,
-	    xm_scalar_t *f) {
-		*f = bb.scalar;
-		return true;
-	});
-	run_jobs("xm_mul", jobs, ELEM_MUL, a->type, a->type, 0, a);
+	auto build = [&]() {
+		return pair_jobs(a, b, ca, cb, [&](const xm_block &bb,
+		    xm_scalar_t *f) {
+			*f = bb.scalar;
+			return true;
+		});
+	};
+	run_jobs("xm_mul", make_key(ELEM_MUL, a, b, idxa, idxb, 0), build,
+	    ELEM_MUL, a->type, a->type, 0, a);
 }
 
 /* libxm src/xm.c:334-413: a = a/(s_b*perm(b)); zero b blocks are fatal. */
@@ -262,15 +406,19 @@ xm_div(xm_tensor_t *a, const xm_tensor_t *b, const char *idxa,
 {
 	xm_dim_t ca, cb;
 	pair_masks("xm_div", a, b, idxa, idxb, true, &ca, &cb);
-	auto jobs = pair_jobs(a, b, ca, cb, [&](const xm_block &bb,
-	    xm_scalar_t *f) {
-		*f = bb.scalar;
-		return true;
-	});
-	for (auto &j : jobs)
-		if (j.src_zero)
-			XM_FATAL("xm_div", "division by zero");
-	run_jobs("xm_div", jobs, ELEM_DIV, a->type, a->type, 0, a);
+	auto build = [&]() {
+		auto jobs = pair_jobs(a, b, ca, cb, [&](const xm_block &bb,
+		    xm_scalar_t *f) {
+			*f = bb.scalar;
+			return true;
+		});
+		for (auto &j : jobs)
+			if (j.src_zero)
+				XM_FATAL("xm_div", "division by zero");
+		return jobs;
+	};
+	run_jobs("xm_div", make_key(ELEM_DIV, a, b, idxa, idxb, 0), build,
+	    ELEM_DIV, a->type, a->type, 0, a);
 }
 
 /* libxm src/xm.c:415-508: sum over all nonzero block pairs of
@@ -343,8 +491,7 @@ xm_dot(const xm_tensor_t *a, const xm_tensor_t *b, const char *idxa,
 		j.src_zero = 0;
 	}
 	if (!gather.empty()) {
-		size_t nc = set_chunks(gather);
-		launch_elem(gather.data(), gather.size(), nc, ELEM_GATHER,
+		launch_elem(gather.data(), gather.size(), ELEM_GATHER,
 		    a->type, a->type, 0, s);
 		HIP_OK("xm_dot", hipStreamSynchronize(s));
 	}

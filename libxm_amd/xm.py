@@ -64,7 +64,9 @@ class ContractStats(C.Structure):
                 ("gemm_launches", C.c_int), ("panel_bytes", C.c_double),
                 ("cin_bytes", C.c_double), ("cout_bytes", C.c_double),
                 ("tile_p", C.c_int), ("tile_q", C.c_int),
-                ("tile_tuning", C.c_int), ("graph", C.c_int)]
+                ("tile_tuning", C.c_int), ("graph", C.c_int),
+                ("permute_tiled", C.c_size_t),
+                ("permute_gather", C.c_size_t)]
 
 
 def dim(seq):
@@ -187,6 +189,7 @@ _SIGNATURES = {
     "xm_amd_fill_random": (None, [_P, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double]),
     "xm_amd_allocator_usage": (None, [_P, C.POINTER(_SZ), C.POINTER(_SZ)]),
     "xm_amd_synchronize": (C.c_int, []),
+    "xm_amd_last_permute": (None, [C.POINTER(_SZ), C.POINTER(_SZ)]),
     "xm_oracle_blas_name": (_S, []),
 }
 
@@ -397,6 +400,12 @@ class XmLib:
         d, h = C.c_size_t(), C.c_size_t()
         self.lib.xm_amd_allocator_usage(alloc, C.byref(d), C.byref(h))
         return d.value, h.value
+
+    def last_permute(self):
+        """(tiled, gather) jobs of the last add/copy/mul/div."""
+        t, g = C.c_size_t(), C.c_size_t()
+        self.lib.xm_amd_last_permute(C.byref(t), C.byref(g))
+        return t.value, g.value
 
 
 HERE = os.path.dirname(os.path.abspath(__file__))
