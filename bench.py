@@ -379,6 +379,43 @@ def permute_mode(lib, args):
     print(json.dumps(out), flush=True)
 
 
+def overflow_mode(lib, args):
+    """The o=64 v=384 ladder with t and c in HBM and v (7.4 GB) in pinned
+    host memory (arena capped at 1 GiB before v is built): one contraction
+    per step, staged (host v blocks copied to HBM on the side stream in
+    batches of C blocks) or, with XM_HOST_STAGING=0, read in place."""
+    from libxm_amd import xm, workloads
+    w = workloads.ladder_d2h(args.o, args.v, args.seed)
+    os.environ["XM_ARENA_BYTES"] = str(1 << 30)
+    alloc = lib.allocator()
+    del os.environ["XM_ARENA_BYTES"]
+    t = w.t.build(lib, alloc)
+    c = w.c.build(lib, alloc)
+    held = []
+    while lib.allocator_usage(alloc)[1] == 0:
+        held.append(lib.lib.xm_allocator_allocate(alloc, 64 << 20))
+    v = w.v.build(lib, alloc)
+    for k, x in enumerate((t, v, c)):
+        fill_tensor(lib, w, x, args.seed * 16 + k)
+    lib.synchronize()
+    dev, host = lib.allocator_usage(alloc)
+    lib.contract(w.alpha, t, v, w.beta, c, *w.idx)
+    lib.synchronize()
+    ms = timed(lib, lambda: lib.contract(w.alpha, t, v, w.beta, c, *w.idx),
+               max(args.steps, 3))
+    st = lib.stats()
+    out = {"mode": "overflow", "workload": w.name + ", v in host memory",
+           "hbm_bytes": dev, "host_bytes": host,
+           "staging": os.environ.get("XM_HOST_STAGING", "1") != "0",
+           "ms_per_call": ms, "tflops": st["flops"] / ms / 1e9,
+           "gemm_ms": st["gemm_ms"], "batches": st["staged_batches"],
+           "staged_bytes": st["staged_bytes"],
+           "staged_gbs": st["staged_bytes"] / ms / 1e6}
+    print(json.dumps(out), flush=True)
+    for p in held:
+        lib.lib.xm_allocator_deallocate(alloc, p)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,7 +429,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--mode", choices=["contract", "copy", "ring", "precision",
-                                       "permute"],
+                                       "permute", "overflow"],
                     default="contract",
                     help="copy: xm_copy of a 4 GiB FP64 tensor, a plain "
                          "copy of known size (HBM counter calibration); "
@@ -400,7 +437,8 @@ def main():
                          "with xm_add / xm_set; precision: the o=64 v=512 "
                          "ladder over six decades in FP64, split and FP32; "
                          "permute: xm_copy under six permutations of 4-D "
-                         "tensors with blocks of 32, 16 and 4-28")
+                         "tensors with blocks of 32, 16 and 4-28; overflow: "
+                         "the ladder with v in pinned host memory")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build runs on a single "
@@ -417,6 +455,8 @@ def main():
         return precision_mode(lib, args)
     if args.mode == "permute":
         return permute_mode(lib, args)
+    if args.mode == "overflow":
+        return overflow_mode(lib, args)
     w = workloads.ladder_d2h(args.o, args.v, args.seed)
     log("workload", json.dumps(w.describe()))
     tb = time.perf_counter()

@@ -82,6 +82,12 @@ typedef struct {
 	/* permute jobs of the call (operand panels, C in, C out) moved by
 	 * the LDS-tiled kernel, and by the gather kernel */
 	size_t permute_tiled, permute_gather;
+	/* contractions touching blocks in pinned host memory (arena overflow)
+	 * run in batches of C blocks, each batch's host operand blocks copied
+	 * into HBM on a side stream while the previous batch computes: the
+	 * batch count (0: nothing staged) and the bytes copied per call */
+	int staged_batches;
+	double staged_bytes;
 } xm_contract_stats_t;
 
 void xm_contract_get_stats(xm_contract_stats_t *stats);

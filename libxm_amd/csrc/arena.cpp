@@ -115,6 +115,21 @@ carve(arena_slab &s, size_t bytes)
 	return XM_NULL_PTR;
 }
 
+uint64_t
+arena_host_slab(xm_allocator_t *a, uint64_t ptr)
+{
+	if (a == nullptr || ptr == XM_NULL_PTR)
+		return 0;
+	std::lock_guard<std::mutex> g(a->mu);
+	if (a->host_bytes == 0)
+		return 0;
+	for (const arena_slab &s : a->slabs)
+		if (s.host && ptr >= (uint64_t)(uintptr_t)s.base &&
+		    ptr < (uint64_t)(uintptr_t)s.base + s.bytes)
+			return (uint64_t)(uintptr_t)s.base;
+	return 0;
+}
+
 XM_API uint64_t
 xm_allocator_allocate(xm_allocator_t *a, size_t size_bytes)
 {

@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <list>
 #include <map>
+#include <set>
 #include <unordered_map>
 
 /* ---- precision selection ----------------------------------------- */
@@ -323,6 +324,10 @@ struct plan_builder {
 	contract_plan *pl;
 	size_t esz;
 	std::map<panel_key, uint64_t> panels;	/* key -> workspace offset */
+	/* staged plans: host-resident source blocks (address -> bytes, slab) */
+	bool stage = false;
+	xm_allocator_t *alloc = nullptr;
+	std::map<uint64_t, std::pair<size_t, uint64_t>> host_blocks;
 
 	/* Operand of one pair: in place, or a (shared) panel. */
 	void operand(const xm_tensor_t *t, size_t flat, const xm_dim_t &F,
@@ -336,7 +341,11 @@ struct plan_builder {
 		uint64_t src = b.type == XM_BLOCK_TYPE_DERIVATIVE ?
 		    t->blocks[b.data].data : b.data;
 		*in_ws = false;
-		if (direct_layout(ls, d, F, Kd, mode, ld)) {
+		/* a host-resident block is never read in place: its staged
+		 * copy is unfolded into a panel */
+		const uint64_t slab = stage ? arena_host_slab(alloc, src) : 0;
+		const bool host = slab != 0;
+		if (!host && direct_layout(ls, d, F, Kd, mode, ld)) {
 			*ptr = src;
 			return;
 		}
@@ -381,6 +390,9 @@ struct plan_builder {
 			j.sstride[k] = key.str[k];
 		}
 		pl->unfold_jobs.push_back(j);
+		pl->unfold_src_ws.push_back(host);
+		if (host)
+			host_blocks[src] = {xm_dim_dot(&d) * esz, slab};
 		pl->unfold_bytes += 2.0 * P * K * esz;
 		pl->panel_bytes += 2.0 * P * K * esz;
 		*ptr = off;
@@ -388,14 +400,25 @@ struct plan_builder {
 	}
 };
 
+/* Host ranges closer than this are staged as one copy. */
+#define STAGE_MERGE_GAP (1u << 20)
+
+/*
+ * The plan of canonical C blocks [lo, hi) (libxm's order).  stage: blocks
+ * in pinned host memory are staged (see contract_plan) rather than read in
+ * place.
+ */
 static contract_plan *
 build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
-    const masks_t &mk, int precision)
+    const masks_t &mk, int precision, size_t lo = 0, size_t hi = SIZE_MAX,
+    bool stage = false)
 {
 	contract_plan *pl = new contract_plan;
 	plan_builder pb;
 	pb.pl = pl;
 	pb.esz = elem_size(c->type);
+	pb.stage = stage;
+	pb.alloc = c->alloc;
 	pl->type = c->type;
 	pl->precision = precision;
 	size_t mult = is_complex(c->type) ? 4 : 1;
@@ -403,11 +426,13 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 	xm_dim_t *list;
 	size_t nlist;
 	xm_tensor_get_canonical_block_list(c, &list, &nlist);
-	pl->nblocks_c = nlist;
+	hi = std::min(hi, nlist);
+	lo = std::min(lo, hi);
+	pl->nblocks_c = hi - lo;
 
 	std::vector<block_pair> all;
 	std::vector<std::vector<uint32_t>> members;
-	for (size_t ic = 0; ic < nlist; ic++) {
+	for (size_t ic = lo; ic < hi; ic++) {
 		const xm_dim_t blkc = list[ic];
 		size_t fc = block_flat(c, blkc);
 		xm_dim_t dc = block_dims_flat(c, fc);
@@ -428,6 +453,11 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 		    contiguous_run(lsc, dc, mk.cidxc, std::max<uint64_t>(n, 1), &e))
 			swap = true;
 		else {
+			swap = false;
+			cpanel = true;
+		}
+		if (stage && arena_is_host(c->alloc, c->blocks[fc].data)) {
+			/* a host C block is computed in an HBM panel */
 			swap = false;
 			cpanel = true;
 		}
@@ -561,6 +591,40 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 	}
 	free(list);
 	pl->nunfold = pl->unfold_jobs.size();
+	if (!pb.host_blocks.empty()) {
+		/* merge host blocks of one slab into ranges, one staged copy
+		 * each, laid out after the panels; point their unfold jobs at
+		 * the copies */
+		std::vector<std::pair<uint64_t, uint64_t>> rng;	/* [start, end) */
+		uint64_t last_slab = 0;
+		for (auto &hb : pb.host_blocks) {
+			uint64_t e = hb.first + hb.second.first;
+			if (!rng.empty() && hb.second.second == last_slab &&
+			    hb.first <= rng.back().second + STAGE_MERGE_GAP)
+				rng.back().second = std::max(rng.back().second, e);
+			else
+				rng.push_back({hb.first, e});
+			last_slab = hb.second.second;
+		}
+		for (auto &r : rng) {
+			uint64_t off = pl->ws_bytes;
+			pl->ws_bytes += (r.second - r.first + 255) / 256 * 256;
+			pl->stage.push_back({r.first, r.second - r.first, off});
+			pl->staged_bytes += (double)(r.second - r.first);
+		}
+		for (size_t i = 0; i < pl->unfold_jobs.size(); i++) {
+			if (!pl->unfold_src_ws[i])
+				continue;
+			elem_job &j = pl->unfold_jobs[i];
+			auto it = std::upper_bound(pl->stage.begin(),
+			    pl->stage.end(), j.src, [](uint64_t v,
+			    const contract_plan::stage_copy &s) {
+				return v < s.host;
+			});
+			--it;
+			j.src = it->off + (j.src - it->host);
+		}
+	}
 	return pl;
 }
 
@@ -587,9 +651,19 @@ plan_release(contract_plan *pl)
 	if (pl == nullptr)
 		return;
 	HIP_OK("xm_contract", hipStreamSynchronize(xm_stream()));
+	HIP_OK("xm_contract", hipStreamSynchronize(xm_side_stream()));
+	for (contract_plan *sub : pl->batches)
+		plan_release(sub);
+	for (void *r : pl->region)
+		if (r)
+			HIP_OK("xm_contract", hipFree(r));
+	for (hipEvent_t e : {pl->ev_start, pl->ev_staged[0], pl->ev_staged[1],
+	    pl->ev_done[0], pl->ev_done[1]})
+		if (e)
+			HIP_OK("xm_contract", hipEventDestroy(e));
 	if (pl->dev)
 		HIP_OK("xm_contract", hipFree(pl->dev));
-	if (pl->ws)
+	if (pl->ws && !pl->ws_external)
 		HIP_OK("xm_contract", hipFree(pl->ws));
 	if (pl->host_scalars)
 		HIP_OK("xm_contract", hipHostFree(pl->host_scalars));
@@ -609,11 +683,14 @@ static void
 plan_upload(contract_plan *pl)
 {
 	const char *fn = "xm_contract";
-	if (pl->ws_bytes)
+	if (pl->ws_bytes && !pl->ws_external)
 		HIP_OK(fn, hipMalloc(&pl->ws, pl->ws_bytes));
 	uint64_t wsb = (uint64_t)(uintptr_t)pl->ws;
-	for (auto &j : pl->unfold_jobs)
-		j.dst += wsb;
+	for (size_t i = 0; i < pl->unfold_jobs.size(); i++) {
+		pl->unfold_jobs[i].dst += wsb;
+		if (pl->unfold_src_ws[i])
+			pl->unfold_jobs[i].src += wsb;
+	}
 	for (auto &j : pl->cin_jobs)
 		j.dst += wsb;
 	for (auto &j : pl->cout_jobs)
@@ -676,6 +753,147 @@ plan_upload(contract_plan *pl)
 		HIP_OK(fn, hipEventCreate(&e));
 }
 
+/* Does the contraction read or write any block in pinned host memory?
+ * (XM_HOST_STAGING=0: treat host blocks like HBM ones, read in place.) */
+static bool
+touches_host(const xm_tensor_t *a, const xm_tensor_t *b,
+    const xm_tensor_t *c)
+{
+	const char *e = getenv("XM_HOST_STAGING");
+	if (e && *e == '0')
+		return false;
+	for (const xm_tensor_t *t : {a, b, c}) {
+		if (t->alloc == nullptr || t->alloc->host_bytes == 0)
+			continue;
+		for (const xm_block &x : t->blocks)
+			if (x.type == XM_BLOCK_TYPE_CANONICAL &&
+			    arena_is_host(t->alloc, x.data))
+				return true;
+	}
+	return false;
+}
+
+/*
+ * Workspace bytes a staging batch aims at: XM_STAGE_BYTES, else a sixth of
+ * the whole contraction's estimate (six batches let the copies of one
+ * overlap the compute of the previous one: on the o=64 v=384 ladder with v
+ * in host memory 6 batches ran at the host link's rate, 1 batch 16% slower,
+ * 15 and 44 batches re-staged shared blocks), at least 256 MiB and at most
+ * a quarter of the free HBM.
+ */
+static size_t
+stage_budget(double total)
+{
+	if (const char *e = getenv("XM_STAGE_BYTES"))
+		return std::max<size_t>(strtoull(e, nullptr, 10), 1);
+	size_t fr = 0, tot = 0;
+	HIP_OK("xm_contract", hipMemGetInfo(&fr, &tot));
+	double b = std::max(total / 6.0, (double)(256ull << 20));
+	return (size_t)std::min(b, (double)fr / 4.0);
+}
+
+/*
+ * Plan of a contraction that touches host blocks: C blocks (libxm's
+ * order) cut into batches whose estimated workspace (host operand blocks
+ * twice: staged copy and panel; other operand blocks once, as an upper
+ * bound on their panels; the C block) stays within the budget, each batch
+ * a plan of its own on one of two alternating workspace regions.  FP64
+ * contractions run in native precision here.
+ */
+static contract_plan *
+build_staged(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
+    const masks_t &mk)
+{
+	const char *fn = "xm_contract";
+	contract_plan *pl = new contract_plan;
+	pl->type = c->type;
+	pl->precision = XM_PRECISION_NATIVE;
+	const size_t esz = elem_size(c->type);
+	xm_dim_t *list;
+	size_t nlist;
+	xm_tensor_get_canonical_block_list(c, &list, &nlist);
+	std::vector<block_pair> all;
+	/* batch boundaries for a budget; *total: the estimate summed over
+	 * the batches */
+	auto partition = [&](double budget, double *total) {
+		std::vector<size_t> cuts = {0};
+		std::set<uint64_t> seen;
+		double est = 0;
+		*total = 0;
+		for (size_t ic = 0; ic < nlist; ic++) {
+			size_t fc = block_flat(c, list[ic]);
+			xm_dim_t dc = block_dims_flat(c, fc);
+			enumerate_pairs(a, b, list[ic], mk, all, nullptr);
+			auto cost = [&](bool fresh) {
+				std::set<uint64_t> s2;
+				double x = (double)xm_dim_dot(&dc) * esz;
+				for (auto &p : all) {
+					if (!p.nonzero || p.alpha == xm_scalar_t(0))
+						continue;
+					for (int w = 0; w < 2; w++) {
+						const xm_tensor_t *t = w ? b : a;
+						size_t f = w ? p.fb : p.fa;
+						const xm_block &x0 = t->blocks[f];
+						uint64_t src = x0.type ==
+						    XM_BLOCK_TYPE_DERIVATIVE ?
+						    t->blocks[x0.data].data : x0.data;
+						if ((!fresh && seen.count(src)) ||
+						    !s2.insert(src).second)
+							continue;
+						xm_dim_t d = block_dims_flat(t, f);
+						double by = (double)xm_dim_dot(&d) *
+						    esz;
+						x += arena_is_host(t->alloc, src) ?
+						    2 * by : by;
+					}
+				}
+				return std::make_pair(x, s2);
+			};
+			auto cs = cost(false);
+			if (ic > cuts.back() && est + cs.first > budget) {
+				cuts.push_back(ic);
+				seen.clear();
+				cs = cost(true);
+				est = 0;
+			}
+			est += cs.first;
+			*total += cs.first;
+			seen.insert(cs.second.begin(), cs.second.end());
+		}
+		cuts.push_back(nlist);
+		return cuts;
+	};
+	double total = 0;
+	partition(1e300, &total);
+	std::vector<size_t> cuts = partition((double)stage_budget(total),
+	    &total);
+	free(list);
+	size_t rb[2] = {0, 0};
+	for (size_t i = 0; i + 1 < cuts.size(); i++) {
+		contract_plan *sub = build_plan(a, b, c, mk, XM_PRECISION_NATIVE,
+		    cuts[i], cuts[i + 1], true);
+		sub->graph_failed = true;	/* direct launches only */
+		sub->ws_external = true;
+		rb[i % 2] = std::max(rb[i % 2], sub->ws_bytes);
+		pl->batches.push_back(sub);
+	}
+	for (int r = 0; r < 2; r++)
+		if (rb[r])
+			HIP_OK(fn, hipMalloc(&pl->region[r], rb[r]));
+	for (size_t i = 0; i < pl->batches.size(); i++) {
+		pl->batches[i]->ws = pl->region[i % 2];
+		plan_upload(pl->batches[i]);
+	}
+	HIP_OK(fn, hipEventCreateWithFlags(&pl->ev_start, hipEventDisableTiming));
+	for (int r = 0; r < 2; r++) {
+		HIP_OK(fn, hipEventCreateWithFlags(&pl->ev_staged[r],
+		    hipEventDisableTiming));
+		HIP_OK(fn, hipEventCreateWithFlags(&pl->ev_done[r],
+		    hipEventDisableTiming));
+	}
+	return pl;
+}
+
 static contract_plan *
 get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
     const char *ia, const char *ib, const char *ic, const masks_t &mk,
@@ -695,6 +913,15 @@ get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 			return cache.front().second;
 		}
 	*cached = false;
+	if (touches_host(a, b, c)) {
+		contract_plan *sp = build_staged(a, b, c, mk);
+		cache.push_front({key, sp});
+		while (cache.size() > CACHE_MAX) {
+			plan_release(cache.back().second);
+			cache.pop_back();
+		}
+		return sp;
+	}
 	contract_plan *pl = build_plan(a, b, c, mk, prec);
 	pl->nslices = key.nslices;
 	plan_upload(pl);
@@ -798,6 +1025,63 @@ run_plan(contract_plan *pl, xm_scalar_t alpha, xm_scalar_t beta)
 	HIP_OK(fn, hipStreamSynchronize(s));
 }
 
+/*
+ * A staged plan's call: batch i computes on the main stream once its host
+ * operand blocks are in region i % 2; the side stream copies batch i + 1's
+ * blocks meanwhile, after batch i - 1 (the region's previous user) is done.
+ */
+static void
+run_staged(contract_plan *pl, xm_scalar_t alpha, xm_scalar_t beta)
+{
+	const char *fn = "xm_contract";
+	hipStream_t s = xm_stream(), side = xm_side_stream();
+	bump_data_epoch();
+	gemm_scalars sc;
+	sc.alpha_re = alpha.real();
+	sc.alpha_im = alpha.imag();
+	sc.beta_re = beta.real();
+	sc.beta_im = beta.imag();
+	sc.flags = alpha == xm_scalar_t(0) ? 1 : 0;
+	sc.flags |= beta == xm_scalar_t(0) ? 2 : 0;
+	const bool pairs = !(sc.flags & 1);
+	const size_t nb = pl->batches.size();
+	for (contract_plan *sub : pl->batches) {
+		*sub->host_scalars = sc;
+		HIP_OK(fn, hipMemcpyAsync(sub->d_scalars, sub->host_scalars,
+		    sizeof sc, hipMemcpyHostToDevice, s));
+		sub->last_graph = false;
+	}
+	auto stage = [&](size_t i) {
+		contract_plan *sub = pl->batches[i];
+		char *base = (char *)sub->ws;
+		for (auto &cp : sub->stage)
+			HIP_OK(fn, hipMemcpyAsync(base + cp.off,
+			    (const void *)(uintptr_t)cp.host, cp.bytes,
+			    hipMemcpyHostToDevice, side));
+		HIP_OK(fn, hipEventRecord(pl->ev_staged[i % 2], side));
+	};
+	if (pairs) {
+		/* the regions may still be read by the previous call */
+		HIP_OK(fn, hipEventRecord(pl->ev_start, s));
+		HIP_OK(fn, hipStreamWaitEvent(side, pl->ev_start, 0));
+		stage(0);
+	}
+	for (size_t i = 0; i < nb; i++) {
+		if (pairs && i + 1 < nb) {
+			if (i >= 1)
+				HIP_OK(fn, hipStreamWaitEvent(side,
+				    pl->ev_done[(i + 1) % 2], 0));
+			stage(i + 1);
+		}
+		if (pairs)
+			HIP_OK(fn, hipStreamWaitEvent(s, pl->ev_staged[i % 2], 0));
+		launch_sequence(pl->batches[i], sc.flags, s, false);
+		HIP_OK(fn, hipEventRecord(pl->ev_done[i % 2], s));
+	}
+	HIP_OK(fn, hipStreamSynchronize(s));
+	HIP_OK(fn, hipStreamSynchronize(side));
+}
+
 /* The plan's device work for one call: permutes in, slices, grouped GEMM,
  * permutes out, with four time marks between the stages: HIP events, or
  * (inside a graph, where events cannot be timed) device wall-clock stamps. */
@@ -842,45 +1126,62 @@ xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
 	contract_masks("xm_contract", a, b, c, idxa, idxb, idxc, mk);
 	bool cached = false;
 	contract_plan *pl = get_plan(a, b, c, idxa, idxb, idxc, mk, &cached);
-	run_plan(pl, alpha, beta);
+	if (pl->batches.empty())
+		run_plan(pl, alpha, beta);
+	else
+		run_staged(pl, alpha, beta);
 	auto t1 = std::chrono::steady_clock::now();
 	xm_contract_stats_t &st = last_stats;
 	memset(&st, 0, sizeof st);
-	st.flops = pl->flops;
-	st.flops_pre = pl->flops_pre;
-	st.npairs = pl->npairs;
-	st.npairs_pre = pl->npairs_pre;
-	st.nblocks_c = pl->nblocks_c;
-	st.nunfold = pl->nunfold;
-	st.unfold_bytes = pl->unfold_bytes;
-	if (pl->last_graph) {
-		const uint64_t *t = stamp_slots();
-		st.prep_ms = stamp_ms(t[0], t[1]);
-		st.gemm_ms = stamp_ms(t[1], t[2]);
-		st.post_ms = stamp_ms(t[2], t[3]);
-	} else {
-		st.prep_ms = elapsed_ms(pl->ev[0], pl->ev[1]);
-		st.gemm_ms = elapsed_ms(pl->ev[1], pl->ev[2]);
-		st.post_ms = elapsed_ms(pl->ev[2], pl->ev[3]);
+	const bool ops = alpha != xm_scalar_t(0), cin = beta != xm_scalar_t(0);
+	/* a staged plan reports the sums over its batches */
+	std::vector<contract_plan *> parts = pl->batches;
+	if (parts.empty())
+		parts.push_back(pl);
+	std::vector<float> gms;
+	for (contract_plan *q : parts) {
+		st.flops += q->flops;
+		st.flops_pre += q->flops_pre;
+		st.npairs += q->npairs;
+		st.npairs_pre += q->npairs_pre;
+		st.nblocks_c += q->nblocks_c;
+		st.nunfold += q->nunfold;
+		st.unfold_bytes += q->unfold_bytes;
+		double ms[3];
+		if (q->last_graph) {
+			const uint64_t *t = stamp_slots();
+			for (int k = 0; k < 3; k++)
+				ms[k] = stamp_ms(t[k], t[k + 1]);
+		} else {
+			for (int k = 0; k < 3; k++)
+				ms[k] = elapsed_ms(q->ev[k], q->ev[k + 1]);
+		}
+		st.prep_ms += ms[0];
+		st.gemm_ms += ms[1];
+		st.post_ms += ms[2];
+		st.gemm_launches += q->gemm_launches;
+		st.panel_bytes += q->panel_bytes;
+		st.cin_bytes += q->cin_bytes;
+		st.cout_bytes += q->cout_bytes;
+		st.permute_tiled += (ops ? q->unfold_l.ntiled : 0) +
+		    (cin ? q->cin_l.ntiled : 0) + q->cout_l.ntiled;
+		st.permute_gather += (ops ? q->unfold_l.ngather : 0) +
+		    (cin ? q->cin_l.ngather : 0) + q->cout_l.ngather;
+		if (ops)
+			st.staged_bytes += q->staged_bytes;
+		gms.push_back((float)ms[1]);
 	}
 	st.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
 	st.plan_cached = cached;
 	st.precision = pl->precision;
 	st.nslices = pl->nslices;
-	st.gemm_launches = pl->gemm_launches;
-	st.panel_bytes = pl->panel_bytes;
-	st.cin_bytes = pl->cin_bytes;
-	st.cout_bytes = pl->cout_bytes;
-	st.tile_p = (int)pl->cand_bp[pl->tile_active];
-	st.tile_q = (int)pl->cand_bq[pl->tile_active];
-	st.tile_tuning = pl->tile_choice < 0;
+	st.tile_p = (int)parts[0]->cand_bp[parts[0]->tile_active];
+	st.tile_q = (int)parts[0]->cand_bq[parts[0]->tile_active];
+	st.tile_tuning = parts[0]->tile_choice < 0;
 	st.graph = pl->last_graph;
-	const bool ops = alpha != xm_scalar_t(0), cin = beta != xm_scalar_t(0);
-	st.permute_tiled = (ops ? pl->unfold_l.ntiled : 0) +
-	    (cin ? pl->cin_l.ntiled : 0) + pl->cout_l.ntiled;
-	st.permute_gather = (ops ? pl->unfold_l.ngather : 0) +
-	    (cin ? pl->cin_l.ngather : 0) + pl->cout_l.ngather;
-	gemm_tile_record(pl, (float)st.gemm_ms);
+	st.staged_batches = (int)pl->batches.size();
+	for (size_t i = 0; i < parts.size(); i++)
+		gemm_tile_record(parts[i], gms[i]);
 }
 
 XM_API void

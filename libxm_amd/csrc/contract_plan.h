@@ -70,6 +70,28 @@ struct contract_plan {
 	/* permute jobs: operand panels, C in, C out */
 	std::vector<elem_job> unfold_jobs, cin_jobs, cout_jobs;
 	elem_launch unfold_l, cin_l, cout_l;	/* set by the upload */
+	std::vector<bool> unfold_src_ws;	/* source is a staged copy */
+
+	/*
+	 * Blocks in pinned host memory (arena overflow).  A plan touching any
+	 * runs as batches of C blocks (`batches`, each a plan of its own
+	 * whose workspace is region[i % 2]); a batch's host operand blocks are
+	 * copied into its workspace on the side stream (`stage`, merged
+	 * address ranges) while the previous batch computes, and unfolded
+	 * from there, so the GEMM reads HBM only.  Host C blocks go through C
+	 * panels.
+	 */
+	struct stage_copy {
+		uint64_t host;		/* host address of a merged range */
+		size_t bytes;
+		uint64_t off;		/* workspace offset of its copy */
+	};
+	std::vector<stage_copy> stage;
+	bool ws_external = false;	/* ws is the parent's region */
+	std::vector<contract_plan *> batches;
+	void *region[2] = {nullptr, nullptr};
+	hipEvent_t ev_start = nullptr, ev_staged[2] = {}, ev_done[2] = {};
+	double staged_bytes = 0;
 
 	/* device memory */
 	size_t ws_bytes = 0;

@@ -1,14 +1,16 @@
 /*
  * device.cpp - the device context: one non-blocking HIP stream at the
- * default priority for all library work, reusable device scratch buffers
- * (descriptor arrays, operand panels) and pinned staging for uploads.
+ * default priority for all library work, a second one (same priority) for
+ * copies that stage host-resident blocks into HBM, reusable device scratch
+ * buffers (descriptor arrays, operand panels) and pinned staging for
+ * uploads.
  */
 #include "internal.h"
 
 #include <cstdlib>
 
 static std::once_flag init_once;
-static hipStream_t stream;
+static hipStream_t stream, side;
 static int device_id;
 
 static void
@@ -20,6 +22,8 @@ init_device(void)
 	HIP_OK("libxm", hipGetDevice(&device_id));
 	HIP_OK("libxm", hipStreamCreateWithFlags(&stream,
 	    hipStreamNonBlocking));
+	HIP_OK("libxm", hipStreamCreateWithFlags(&side,
+	    hipStreamNonBlocking));
 }
 
 hipStream_t
@@ -27,6 +31,13 @@ xm_stream(void)
 {
 	std::call_once(init_once, init_device);
 	return stream;
+}
+
+hipStream_t
+xm_side_stream(void)
+{
+	std::call_once(init_once, init_device);
+	return side;
 }
 
 struct scratch_slot {
@@ -131,5 +142,6 @@ XM_API int
 xm_amd_synchronize(void)
 {
 	HIP_OK("xm_amd_synchronize", hipStreamSynchronize(xm_stream()));
+	HIP_OK("xm_amd_synchronize", hipStreamSynchronize(xm_side_stream()));
 	return 0;
 }

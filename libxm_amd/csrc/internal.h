@@ -105,6 +105,16 @@ void bump_data_epoch(void);
 /* ---- device context (device.cpp) ----------------------------------- */
 
 hipStream_t xm_stream(void);
+/* Stream for host -> HBM staging copies (default priority). */
+hipStream_t xm_side_stream(void);
+/* Base address of the allocator's pinned host slab holding `ptr`, or 0
+ * when `ptr` is not in host memory. */
+uint64_t arena_host_slab(xm_allocator_t *a, uint64_t ptr);
+static inline bool
+arena_is_host(xm_allocator_t *a, uint64_t ptr)
+{
+	return arena_host_slab(a, ptr) != 0;
+}
 /* Reusable device scratch of at least `bytes`, grown on demand; slot
  * selects independent buffers.  Valid until the next call with the same
  * slot. */

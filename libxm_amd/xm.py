@@ -66,7 +66,8 @@ class ContractStats(C.Structure):
                 ("tile_p", C.c_int), ("tile_q", C.c_int),
                 ("tile_tuning", C.c_int), ("graph", C.c_int),
                 ("permute_tiled", C.c_size_t),
-                ("permute_gather", C.c_size_t)]
+                ("permute_gather", C.c_size_t),
+                ("staged_batches", C.c_int), ("staged_bytes", C.c_double)]
 
 
 def dim(seq):
