@@ -167,6 +167,87 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, split_slices=None):
             "rel_err_split_gpu_vs_cpu": errs.get(xm.PRECISION_SPLIT_BF16)}
 
 
+def cpu_ring_sample(lib, w, galloc, t, v, c, c2, frac=0.1):
+    """libxm on the host cores (or the oracle, if it is absent) beside the
+    GPU for the ring: the first canonical C blocks holding about `frac` of
+    the FLOPs, contracted from the same operand blocks, then xm_add and
+    xm_set on those C blocks; each also run on the GPU on the same sample
+    (contraction within 1e-13, element operations bit for bit)."""
+    import numpy as np
+    from libxm_amd import xm
+    from libxm_amd.spec import Spec, rel_err
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libxmref.so")
+    port_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    kind = "reference" if os.path.exists(ref_so) else "port"
+    os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+    cpu = xm.XmLib(ref_so if kind == "reference" else port_so)
+    ncores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    pairs_all = lib.get_pairs(t, v, c, *w.idx)
+    f_of = {}
+    for p in pairs_all:
+        f_of[p["c"]] = f_of.get(p["c"], 0) + 2.0 * p["m"] * p["n"] * p["k"]
+    target = frac * sum(f_of.values())
+    chosen, fsum = [], 0.0
+    for idx in w.c.canonical_blocks():
+        if chosen and fsum + f_of.get(idx, 0) > target:
+            break
+        chosen.append(idx)
+        fsum += f_of.get(idx, 0)
+    sel = set(chosen)
+    need_a = {p["a"] for p in pairs_all if p["c"] in sel}
+    need_b = {p["b"] for p in pairs_all if p["c"] in sel}
+    cs = Spec(w.c.dims, {k: p for k, p in enumerate(w.c.points)}, w.c.type)
+    for idx in chosen:
+        cs.canon(idx)
+    alloc = cpu.allocator()
+    ct, cv = w.specs[0].build(cpu, alloc), w.specs[1].build(cpu, alloc)
+    cc, cc2 = cs.build(cpu, alloc), cs.build(cpu, alloc)
+    for i in need_a:
+        cpu.write_block(ct, i, lib.read_block(t, i))
+    for i in need_b:
+        cpu.write_block(cv, i, lib.read_block(v, i))
+    c0 = {i: lib.read_block(c, i) for i in chosen}
+    d2 = {i: lib.read_block(c2, i) for i in chosen}
+    gc, gc2 = cs.build(lib, galloc), cs.build(lib, galloc)
+    out = {"kind": kind, "cores": ncores,
+           "sample": "%d of %d canonical C blocks, %.3g GFLOP, %.2f GB of C" % (
+               len(chosen), len(w.c.canonical_blocks()), fsum / 1e9,
+               cs.canonical_bytes() / 1e9)}
+    for x, y, d in ((cc, gc, c0), (cc2, gc2, d2)):
+        for i in chosen:
+            cpu.write_block(x, i, d[i])
+            lib.write_block(y, i, d[i])
+    t0 = time.perf_counter()
+    cpu.contract(w.alpha, ct, cv, w.beta, cc, *w.idx)
+    tc = time.perf_counter() - t0
+    lib.contract(w.alpha, t, v, w.beta, gc, *w.idx)
+    out["contract"] = {"tflops": fsum / tc / 1e12, "seconds": tc,
+                       "rel_err_gpu_vs_cpu": rel_err(cs.read(lib, gc),
+                                                     cs.read(cpu, cc))}
+    cb = cs.canonical_bytes()
+    for i in chosen:        # both from the same C again
+        cpu.write_block(cc, i, c0[i])
+        lib.write_block(gc, i, c0[i])
+    t0 = time.perf_counter()
+    cpu.add(1.0, cc, 0.5, cc2, "ijab", "ijab")
+    ta = time.perf_counter() - t0
+    lib.add(1.0, gc, 0.5, gc2, "ijab", "ijab")
+    ga, ca = cs.read(lib, gc), cs.read(cpu, cc)
+    t0 = time.perf_counter()
+    cpu.set(cc, 0.0)
+    tsv = time.perf_counter() - t0
+    out["xm_add"] = {"gbs": 3 * cb / ta / 1e9, "seconds": ta,
+                     "bit_exact": all(np.array_equal(ga[k], ca[k])
+                                      for k in ca)}
+    out["xm_set"] = {"gbs": cb / tsv / 1e9, "seconds": tsv}
+    for x in (ct, cv, cc, cc2):
+        cpu.tensor_free(x)
+    cpu.allocator_destroy(alloc)
+    for x in (gc, gc2):
+        lib.tensor_free(x)
+    return out
+
+
 def split_accuracy(lib, w, alloc, t, v, nslices):
     """Product term alone (beta = 0): split vs FP64 on the whole workload."""
     from libxm_amd import xm
@@ -223,7 +304,9 @@ def timed(lib, fn, steps):
 def ring_mode(lib, args):
     """Ring c(i,j,a,b) = sum t(i,k,a,c) w(k,b,c,j), o=96 v=480, block sizes
     U[4,28] (~2.5e5 block pairs, unfolds of both operands), FP64 and
-    complex64; then xm_add(1, c, 0.5, c2) and xm_set(c, 0) on the result."""
+    complex64; then xm_add(1, c, 0.5, c2) and xm_set(c, 0) on the result.
+    The FP64 form also times libxm on the host cores on a sample of C
+    blocks (contraction, xm_add, xm_set), unless --no-cpu."""
     from libxm_amd import xm, workloads
     out = {"mode": "ring", "unit": "TFLOP/s", "forms": {}}
     for name, tp, esize, mult in (("f64", xm.SCALAR_DOUBLE, 8, 2.0),
@@ -266,6 +349,8 @@ def ring_mode(lib, args):
                        "frac_of_8tbs": 3 * cb / add_ms / 1e6 / HBM_PEAK}
         f["xm_set"] = {"ms": set_ms, "bytes": cb, "gbs": cb / set_ms / 1e6,
                        "frac_of_8tbs": cb / set_ms / 1e6 / HBM_PEAK}
+        if name == "f64" and not args.no_cpu:
+            f["cpu_baseline"] = cpu_ring_sample(lib, w, alloc, t, v, c, c2)
         log("ring %s: %.3f ms/call %.2f TFLOP/s (GEMM %.3f ms), unfold %.1f GB "
             "%.3f ms, fold %.1f GB %.3f ms, add %.1f GB/s, set %.1f GB/s" % (
                 name, ms, F / ms / 1e9, gemm_ms, unf / 1e9, prep_ms,
@@ -376,7 +461,35 @@ def permute_mode(lib, args):
         lib.tensor_free(a)
         lib.tensor_free(b)
         lib.allocator_destroy(alloc)
+        if name == cases[0][0] and not args.no_cpu:
+            out["cpu_baseline"] = cpu_permute(sp)
     print(json.dumps(out), flush=True)
+
+
+def cpu_permute(sp):
+    """libxm's xm_copy on the host cores (or the oracle's) for the same
+    tensors and permutations, one run each after a warm-up run."""
+    from libxm_amd import xm
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libxmref.so")
+    port_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    kind = "reference" if os.path.exists(ref_so) else "port"
+    cpu = xm.XmLib(ref_so if kind == "reference" else port_so)
+    alloc = cpu.allocator()
+    a, b = sp.build(cpu, alloc), sp.build(cpu, alloc)
+    cpu.set(a, 1.0)
+    cpu.copy(b, 1.0, a, "ijkl", "ijkl")
+    nbytes = 2 * sp.canonical_bytes()
+    res = {"kind": kind, "cores": int(os.environ.get("OMP_NUM_THREADS",
+                                                     os.cpu_count() or 1)),
+           "tensor": "x".join(str(d) for d in sp.dims), "gbs": {}}
+    for perm in ("ijkl", "jikl", "ikjl", "lkji", "klij", "ljki"):
+        t0 = time.perf_counter()
+        cpu.copy(b, 1.0, a, "ijkl", perm)
+        res["gbs"][perm] = nbytes / (time.perf_counter() - t0) / 1e9
+    cpu.tensor_free(a)
+    cpu.tensor_free(b)
+    cpu.allocator_destroy(alloc)
+    return res
 
 
 def overflow_mode(lib, args):
