@@ -345,14 +345,20 @@ struct plan_builder {
 		 * copy is unfolded into a panel */
 		const uint64_t slab = stage ? arena_host_slab(alloc, src) : 0;
 		const bool host = slab != 0;
-		if (!host && direct_layout(ls, d, F, Kd, mode, ld)) {
+		/* FP32 / complex64 plans want P-contiguous panels: their MFMA
+		 * kernel reads only those */
+		const bool pfirst = pl->type == XM_SCALAR_FLOAT ||
+		    pl->type == XM_SCALAR_FLOAT_COMPLEX;
+		if (!host && direct_layout(ls, d, F, Kd, mode, ld) &&
+		    !(pfirst && *mode == PANEL_KCONTIG)) {
 			*ptr = src;
 			return;
 		}
-		/* Permuted panel; P-contiguous unless the source's
-		 * stride-1 index is a contracted one. */
+		/* Permuted panel; P-contiguous unless the source's stride-1
+		 * index is a contracted one and the GEMM reads either layout
+		 * (the permute then reads and writes in runs). */
 		bool kfirst = false;
-		for (size_t k = 0; k < Kd.n; k++)
+		for (size_t k = 0; k < Kd.n && !pfirst; k++)
 			if (ls[Kd.i[k]] == 1 && d.i[Kd.i[k]] > 1)
 				kfirst = true;
 		panel_key key;

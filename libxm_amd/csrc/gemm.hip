@@ -513,6 +513,209 @@ k_gemm_simple(const gemm_tile *__restrict__ tiles,
 		}
 }
 
+/* ---- FP32 and complex64 on MFMA ------------------------------------ */
+
+/*
+ * FP32 / complex64 tile kernel for plans whose every X and Y panel is
+ * P-contiguous: 64 x 64 outputs, 4 waves as 2 x 2 of 32 x 32, each a 2 x 2
+ * set of v_mfma_f32_16x16x4_f32 blocks.  Operands move global -> LDS by
+ * dword global_load_lds (lane l moves word l of a 64-word run of the stage,
+ * laid out [k][row][re, im]; any extents and alignment), two stages with
+ * the DMA one 16-k step ahead, a counted wait and a bare s_barrier per step
+ * (the FP64 kernel's scheme).  A complex product takes four real MFMAs into
+ * a real and an imaginary accumulator: re += yr xr - yi xi, im += yr xi +
+ * yi xr.  The pair factor multiplies X in registers.  The FP32 MFMA leaves
+ * row 4 (lane / 16) + r, column lane % 16 in register r (checked on the
+ * device), so with Y as the A operand P runs along the lanes.
+ */
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct m32 { static constexpr int W = 1; };
+template <> struct m32<cx<float>> { static constexpr int W = 2; };
+
+#define M32_B 64
+#define M32_BK 16
+
+template <typename T>
+__global__ void __launch_bounds__(GEMM_WG, 4)
+k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
+    const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
+    const gemm_scalars *__restrict__ scp)
+{
+	constexpr int W = m32<T>::W;		/* floats per element */
+	constexpr bool CX = W == 2;
+	constexpr int COLW = M32_B * W;		/* words of one 64-row column */
+	constexpr int SW = M32_BK * COLW;	/* words of one operand stage */
+	constexpr int ND = SW / 64;		/* DMAs per operand per step */
+	static_assert(ND % 4 == 0, "DMAs per wave");
+	__shared__ float st[2][2 * SW];		/* [stage][X, Y][k][row][re, im] */
+
+	const gemm_tile tl = tiles[blockIdx.x];
+	if (tl.cblock == 0xffffffffu)
+		return;
+	const gemm_cblock cb = cbs[tl.cblock];
+	const gemm_scalars sc = *scp;
+	const int tid = threadIdx.x, lane = tid & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+	const int wp = (wave & 1) * 32, wq = (wave >> 1) * 32;
+	const bool live = tl.p0 + wp < cb.p && tl.q0 + wq < cb.q;
+	const uint32_t pend = cb.pair_end;
+	const uint32_t pbeg = (sc.flags & 1) ? pend : cb.pair_begin;
+
+	f4 acc[2][2][W];
+#pragma unroll
+	for (int i = 0; i < 2; i++)
+#pragma unroll
+		for (int j = 0; j < 2; j++)
+#pragma unroll
+			for (int c = 0; c < W; c++)
+				acc[i][j][c] = f4{0, 0, 0, 0};
+
+	/* step at column k0 of pair px into stage nx (X words, then Y words
+	 * at SW): DMA d of an operand moves its words [64 d, 64 d + 64); wave
+	 * w issues d = w, w + 4, ... */
+	auto fetch = [&](float *__restrict__ nx, const gemm_pair px,
+	    uint32_t k0) {
+		const float *X = (const float *)(uintptr_t)px.x;
+		const float *Y = (const float *)(uintptr_t)px.y;
+#pragma unroll
+		for (int op = 0; op < 2; op++) {
+			const float *Z = op ? Y : X;
+			const uint32_t ld = op ? px.ldy : px.ldx;
+			const uint32_t z0 = op ? tl.q0 : tl.p0;
+			const uint32_t zn = op ? cb.q : cb.p;
+#pragma unroll
+			for (int d = 0; d < ND / 4; d++) {
+				const int dd = wave + 4 * d, e = dd * 64 + lane;
+				const uint32_t k = k0 + e / COLW;
+				const uint32_t r = z0 + (e % COLW) / W;
+				const void *src = (const float *)fd_zero_page + lane;
+				if (k < px.k && r < zn)
+					src = Z + ((size_t)r + (size_t)k * ld) * W +
+					    e % W;
+				__builtin_amdgcn_global_load_lds((fd_gbl_t *)src,
+				    (fd_lds_t *)&nx[op * SW + dd * 64], 4, 0, 0);
+			}
+		}
+	};
+	auto advance = [&](uint32_t &i, uint32_t &k, gemm_pair &p) {
+		k += M32_BK;
+		if (k >= p.k) {
+			k = 0;
+			if (++i < pend)
+				p = pairs[i];
+		}
+	};
+	uint32_t ip = pbeg, kc = 0, fip = pbeg, fk = 0;
+	gemm_pair pr, fpr;
+	if (ip < pend) {
+		pr = pairs[ip];
+		fpr = pr;
+		fetch(st[0], fpr, fk);
+		advance(fip, fk, fpr);
+	}
+	__builtin_amdgcn_s_waitcnt(0);
+	__syncthreads();
+	/* the stages reach step() as restrict pointers, so the LDS reads of
+	 * this stage need not wait for the DMA into the other */
+	auto step = [&](const float *__restrict__ cur, float *__restrict__ nxt) {
+		const bool more = fip < pend;
+		if (more) {
+			fetch(nxt, fpr, fk);
+			advance(fip, fk, fpr);
+		}
+		if (live) {
+			const float ar = (float)(sc.alpha_re * pr.a_re -
+			    sc.alpha_im * pr.a_im);
+			const float ai = (float)(sc.alpha_re * pr.a_im +
+			    sc.alpha_im * pr.a_re);
+#pragma unroll
+			for (int kk = 0; kk < M32_BK; kk += 4) {
+				const int kr = kk + (lane >> 4), c16 = lane & 15;
+				float xa[2][W], yb[2][W];
+#pragma unroll
+				for (int i = 0; i < 2; i++) {
+					const float *x = cur + kr * COLW +
+					    (wp + i * 16 + c16) * W;
+					if (CX) {
+						const float xr = x[0], xi = x[W - 1];
+						xa[i][0] = xr * ar - xi * ai;
+						xa[i][W - 1] = xr * ai + xi * ar;
+					} else {
+						xa[i][0] = x[0] * ar;
+					}
+				}
+#pragma unroll
+				for (int j = 0; j < 2; j++) {
+					const float *y = cur + SW + kr * COLW +
+					    (wq + j * 16 + c16) * W;
+#pragma unroll
+					for (int c = 0; c < W; c++)
+						yb[j][c] = y[c];
+				}
+#pragma unroll
+				for (int i = 0; i < 2; i++)
+#pragma unroll
+					for (int j = 0; j < 2; j++) {
+						acc[i][j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+						    yb[j][0], xa[i][0], acc[i][j][0], 0, 0, 0);
+						if (CX) {
+							acc[i][j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+							    -yb[j][W - 1], xa[i][W - 1],
+							    acc[i][j][0], 0, 0, 0);
+							acc[i][j][W - 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+							    yb[j][0], xa[i][W - 1],
+							    acc[i][j][W - 1], 0, 0, 0);
+							acc[i][j][W - 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+							    yb[j][W - 1], xa[i][0],
+							    acc[i][j][W - 1], 0, 0, 0);
+						}
+					}
+			}
+		}
+		/* this stage is read and this wave's DMAs of the next step have
+		 * landed; a bare s_barrier then publishes the next stage */
+		__builtin_amdgcn_s_waitcnt(0);
+		__atomic_signal_fence(__ATOMIC_SEQ_CST);
+		__builtin_amdgcn_s_barrier();
+		__atomic_signal_fence(__ATOMIC_SEQ_CST);
+		advance(ip, kc, pr);
+	};
+	int cs = 0;
+	while (ip < pend) {
+		step(st[cs], st[cs ^ 1]);
+		cs ^= 1;
+	}
+
+	typedef ops<T> O;
+	T *C = (T *)(uintptr_t)cb.c;
+	const T beta = O::mk(sc.beta_re, sc.beta_im);
+	const bool bz = sc.flags & 2, none = pbeg == pend;
+#pragma unroll
+	for (int i = 0; i < 2; i++)
+#pragma unroll
+		for (int j = 0; j < 2; j++)
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const uint32_t p = tl.p0 + wp + i * 16 + (lane & 15);
+				const uint32_t q = tl.q0 + wq + j * 16 +
+				    4 * (lane >> 4) + r;
+				if (p < cb.p && q < cb.q) {
+					const size_t o = (size_t)p + (size_t)q * cb.ldc;
+					const T a = O::mk(acc[i][j][0][r],
+					    acc[i][j][W - 1][r]);
+					T v;
+					if (bz)
+						v = none ? O::zero() : a;
+					else if (none)
+						v = O::mul(beta, C[o]);
+					else
+						v = O::add(O::mul(beta, C[o]), a);
+					C[o] = v;
+				}
+			}
+}
+
 /* ---- tiles and launch --------------------------------------------- */
 
 static bool
@@ -658,10 +861,25 @@ f64_dma_eligible(const contract_plan *pl)
 	return true;
 }
 
+/* Every X and Y panel P-contiguous in an FP32 or complex64 plan: what
+ * k_gemm_mfma32 reads.  XM_GEMM_SIMPLE=1 keeps the FMA kernel. */
+static bool
+f32_mfma_eligible(const contract_plan *pl)
+{
+	if ((pl->type != XM_SCALAR_FLOAT && pl->type != XM_SCALAR_FLOAT_COMPLEX) ||
+	    use_simple())
+		return false;
+	for (const gemm_pair &g : pl->pairs)
+		if (g.xmode != PANEL_PCONTIG || g.ymode != PANEL_PCONTIG)
+			return false;
+	return true;
+}
+
 void
 build_tiles(contract_plan *pl)
 {
 	pl->f64_dma = f64_dma_eligible(pl);
+	pl->f32_mfma = f32_mfma_eligible(pl);
 	std::vector<std::pair<uint32_t, uint32_t>> cand;
 	tile_candidates(pl, cand);
 	pl->ncand = (int)std::min<size_t>(cand.size(), MAX_TILE_CANDIDATES);
@@ -743,12 +961,20 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_FLOAT:
-		k_gemm_simple<float><<<g, b, 0, s>>>(pl->d_tiles, pl->d_cblocks,
-		    pl->d_pairs, pl->d_scalars);
+		if (pl->f32_mfma)
+			k_gemm_mfma32<float><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		else
+			k_gemm_simple<float><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_FLOAT_COMPLEX:
-		k_gemm_simple<cx<float>><<<g, b, 0, s>>>(pl->d_tiles,
-		    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		if (pl->f32_mfma)
+			k_gemm_mfma32<cx<float>><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		else
+			k_gemm_simple<cx<float>><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_DOUBLE_COMPLEX:
 		k_gemm_simple<cx<double>><<<g, b, 0, s>>>(pl->d_tiles,
