@@ -23,6 +23,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <type_traits>
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -533,10 +534,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 template <typename T> struct m32 { static constexpr int W = 1; };
 template <> struct m32<cx<float>> { static constexpr int W = 2; };
 
-#define M32_B 64
 #define M32_BK 16
 
-template <typename T>
+template <typename T, int BQ>
 __global__ void __launch_bounds__(GEMM_WG, 4)
 k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
     const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
@@ -544,11 +544,11 @@ k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
 {
 	constexpr int W = m32<T>::W;		/* floats per element */
 	constexpr bool CX = W == 2;
-	constexpr int COLW = M32_B * W;		/* words of one 64-row column */
-	constexpr int SW = M32_BK * COLW;	/* words of one operand stage */
-	constexpr int ND = SW / 64;		/* DMAs per operand per step */
-	static_assert(ND % 4 == 0, "DMAs per wave");
-	__shared__ float st[2][2 * SW];		/* [stage][X, Y][k][row][re, im] */
+	constexpr int NT = BQ / 32;		/* 16-column blocks per wave */
+	constexpr int XW = 64 * W, YW = BQ * W;	/* words of one column */
+	constexpr int SX = M32_BK * XW, SY = M32_BK * YW;	/* of a stage */
+	static_assert(SX % 256 == 0 && SY % 256 == 0, "DMAs per wave");
+	__shared__ float st[2][SX + SY];	/* [stage][X, Y][k][row][re, im] */
 
 	const gemm_tile tl = tiles[blockIdx.x];
 	if (tl.cblock == 0xffffffffu)
@@ -557,46 +557,47 @@ k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
 	const gemm_scalars sc = *scp;
 	const int tid = threadIdx.x, lane = tid & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-	const int wp = (wave & 1) * 32, wq = (wave >> 1) * 32;
+	const int wp = (wave & 1) * 32, wq = (wave >> 1) * (BQ / 2);
 	const bool live = tl.p0 + wp < cb.p && tl.q0 + wq < cb.q;
 	const uint32_t pend = cb.pair_end;
 	const uint32_t pbeg = (sc.flags & 1) ? pend : cb.pair_begin;
 
-	f4 acc[2][2][W];
+	f4 acc[2][NT][W];
 #pragma unroll
 	for (int i = 0; i < 2; i++)
 #pragma unroll
-		for (int j = 0; j < 2; j++)
+		for (int j = 0; j < NT; j++)
 #pragma unroll
 			for (int c = 0; c < W; c++)
 				acc[i][j][c] = f4{0, 0, 0, 0};
 
 	/* step at column k0 of pair px into stage nx (X words, then Y words
-	 * at SW): DMA d of an operand moves its words [64 d, 64 d + 64); wave
+	 * at SX): DMA d of an operand moves its words [64 d, 64 d + 64); wave
 	 * w issues d = w, w + 4, ... */
+	auto fetch_op = [&](float *__restrict__ nz, const float *Z, uint32_t ld,
+	    uint32_t z0, uint32_t zn, uint32_t kn, uint32_t k0, auto cw,
+	    auto sw) {
+		constexpr int CW = decltype(cw)::value, SWD = decltype(sw)::value;
+#pragma unroll
+		for (int d = 0; d < SWD / 256; d++) {
+			const int dd = wave + 4 * d, e = dd * 64 + lane;
+			const uint32_t k = k0 + e / CW;
+			const uint32_t r = z0 + (e % CW) / W;
+			const void *src = (const float *)fd_zero_page + lane;
+			if (k < kn && r < zn)
+				src = Z + ((size_t)r + (size_t)k * ld) * W + e % W;
+			__builtin_amdgcn_global_load_lds((fd_gbl_t *)src,
+			    (fd_lds_t *)&nz[dd * 64], 4, 0, 0);
+		}
+	};
 	auto fetch = [&](float *__restrict__ nx, const gemm_pair px,
 	    uint32_t k0) {
-		const float *X = (const float *)(uintptr_t)px.x;
-		const float *Y = (const float *)(uintptr_t)px.y;
-#pragma unroll
-		for (int op = 0; op < 2; op++) {
-			const float *Z = op ? Y : X;
-			const uint32_t ld = op ? px.ldy : px.ldx;
-			const uint32_t z0 = op ? tl.q0 : tl.p0;
-			const uint32_t zn = op ? cb.q : cb.p;
-#pragma unroll
-			for (int d = 0; d < ND / 4; d++) {
-				const int dd = wave + 4 * d, e = dd * 64 + lane;
-				const uint32_t k = k0 + e / COLW;
-				const uint32_t r = z0 + (e % COLW) / W;
-				const void *src = (const float *)fd_zero_page + lane;
-				if (k < px.k && r < zn)
-					src = Z + ((size_t)r + (size_t)k * ld) * W +
-					    e % W;
-				__builtin_amdgcn_global_load_lds((fd_gbl_t *)src,
-				    (fd_lds_t *)&nx[op * SW + dd * 64], 4, 0, 0);
-			}
-		}
+		fetch_op(nx, (const float *)(uintptr_t)px.x, px.ldx, tl.p0, cb.p,
+		    px.k, k0, std::integral_constant<int, XW>(),
+		    std::integral_constant<int, SX>());
+		fetch_op(nx + SX, (const float *)(uintptr_t)px.y, px.ldy, tl.q0,
+		    cb.q, px.k, k0, std::integral_constant<int, YW>(),
+		    std::integral_constant<int, SY>());
 	};
 	auto advance = [&](uint32_t &i, uint32_t &k, gemm_pair &p) {
 		k += M32_BK;
@@ -632,10 +633,10 @@ k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
 #pragma unroll
 			for (int kk = 0; kk < M32_BK; kk += 4) {
 				const int kr = kk + (lane >> 4), c16 = lane & 15;
-				float xa[2][W], yb[2][W];
+				float xa[2][W], yb[NT][W];
 #pragma unroll
 				for (int i = 0; i < 2; i++) {
-					const float *x = cur + kr * COLW +
+					const float *x = cur + kr * XW +
 					    (wp + i * 16 + c16) * W;
 					if (CX) {
 						const float xr = x[0], xi = x[W - 1];
@@ -646,8 +647,8 @@ k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
 					}
 				}
 #pragma unroll
-				for (int j = 0; j < 2; j++) {
-					const float *y = cur + SW + kr * COLW +
+				for (int j = 0; j < NT; j++) {
+					const float *y = cur + SX + kr * YW +
 					    (wq + j * 16 + c16) * W;
 #pragma unroll
 					for (int c = 0; c < W; c++)
@@ -656,7 +657,7 @@ k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
 #pragma unroll
 				for (int i = 0; i < 2; i++)
 #pragma unroll
-					for (int j = 0; j < 2; j++) {
+					for (int j = 0; j < NT; j++) {
 						acc[i][j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(
 						    yb[j][0], xa[i][0], acc[i][j][0], 0, 0, 0);
 						if (CX) {
@@ -694,7 +695,7 @@ k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
 #pragma unroll
 	for (int i = 0; i < 2; i++)
 #pragma unroll
-		for (int j = 0; j < 2; j++)
+		for (int j = 0; j < NT; j++)
 #pragma unroll
 			for (int r = 0; r < 4; r++) {
 				const uint32_t p = tl.p0 + wp + i * 16 + (lane & 15);
@@ -749,6 +750,17 @@ tile_candidates(const contract_plan *pl, std::vector<std::pair<uint32_t,
 		out.push_back(split_staged_kernel(pl->nslices) ?
 		    std::make_pair(128u, 128u) : std::make_pair(64u, 64u));
 	} else if (pl->type == XM_SCALAR_DOUBLE && !use_simple()) {
+		const char *e = getenv("XM_GEMM_TILE");
+		if (e && strcmp(e, "64x64") == 0)
+			out.push_back({64, 64});
+		else if (e && strcmp(e, "64x128") == 0)
+			out.push_back({64, 128});
+		else {
+			out.push_back({64, 128});
+			out.push_back({64, 64});
+		}
+	} else if (pl->f32_mfma && pl->type == XM_SCALAR_FLOAT) {
+		/* (complex64 at 64 x 128 would hold 259 registers: 1 wave) */
 		const char *e = getenv("XM_GEMM_TILE");
 		if (e && strcmp(e, "64x64") == 0)
 			out.push_back({64, 64});
@@ -961,8 +973,11 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_FLOAT:
-		if (pl->f32_mfma)
-			k_gemm_mfma32<float><<<g, b, 0, s>>>(pl->d_tiles,
+		if (pl->f32_mfma && pl->cand_bq[pl->tile_active] == 128)
+			k_gemm_mfma32<float, 128><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		else if (pl->f32_mfma)
+			k_gemm_mfma32<float, 64><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		else
 			k_gemm_simple<float><<<g, b, 0, s>>>(pl->d_tiles,
@@ -970,7 +985,7 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 		break;
 	case XM_SCALAR_FLOAT_COMPLEX:
 		if (pl->f32_mfma)
-			k_gemm_mfma32<cx<float>><<<g, b, 0, s>>>(pl->d_tiles,
+			k_gemm_mfma32<cx<float>, 64><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		else
 			k_gemm_simple<cx<float>><<<g, b, 0, s>>>(pl->d_tiles,
