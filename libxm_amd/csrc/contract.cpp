@@ -345,10 +345,11 @@ struct plan_builder {
 		 * copy is unfolded into a panel */
 		const uint64_t slab = stage ? arena_host_slab(alloc, src) : 0;
 		const bool host = slab != 0;
-		/* FP32 / complex64 plans want P-contiguous panels: their MFMA
+		/* FP32 and complex plans want P-contiguous panels: their MFMA
 		 * kernel reads only those */
 		const bool pfirst = pl->type == XM_SCALAR_FLOAT ||
-		    pl->type == XM_SCALAR_FLOAT_COMPLEX;
+		    pl->type == XM_SCALAR_FLOAT_COMPLEX ||
+		    pl->type == XM_SCALAR_DOUBLE_COMPLEX;
 		if (!host && direct_layout(ls, d, F, Kd, mode, ld) &&
 		    !(pfirst && *mode == PANEL_KCONTIG)) {
 			*ptr = src;

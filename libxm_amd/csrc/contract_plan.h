@@ -65,7 +65,7 @@ struct contract_plan {
 	int tile_next = 0;		/* calls of the tuning sequence so far */
 	int tile_active = 0;		/* candidate of the last launch */
 	bool f64_dma = false;		/* k_gemm_f64_dma can run the 64 x 128 tiles */
-	bool f32_mfma = false;		/* k_gemm_mfma32 can run the plan */
+	bool mfma_dw = false;		/* k_gemm_mfma_dw can run the plan */
 	std::string tile_sig;		/* block-size histogram signature */
 
 	/* permute jobs: operand panels, C in, C out */

@@ -514,41 +514,58 @@ k_gemm_simple(const gemm_tile *__restrict__ tiles,
 		}
 }
 
-/* ---- FP32 and complex64 on MFMA ------------------------------------ */
+/* ---- FP32, complex64 and complex128 on MFMA ------------------------ */
 
 /*
- * FP32 / complex64 tile kernel for plans whose every X and Y panel is
- * P-contiguous: 64 x 64 outputs, 4 waves as 2 x 2 of 32 x 32, each a 2 x 2
- * set of v_mfma_f32_16x16x4_f32 blocks.  Operands move global -> LDS by
+ * FP32 / complex64 / complex128 tile kernel for plans whose every X and Y
+ * panel is P-contiguous: 64 x BQ outputs, 4 waves as 2 x 2 of 32 x BQ/2,
+ * each a set of 16 x 16 x 4 MFMA blocks (v_mfma_f32_16x16x4_f32, or
+ * v_mfma_f64_16x16x4_f64 for complex128).  Operands move global -> LDS by
  * dword global_load_lds (lane l moves word l of a 64-word run of the stage,
  * laid out [k][row][re, im]; any extents and alignment), two stages with
- * the DMA one 16-k step ahead, a counted wait and a bare s_barrier per step
+ * the DMA one M32_BK-k step ahead (16, 8 for complex128), a counted wait and a bare s_barrier per step
  * (the FP64 kernel's scheme).  A complex product takes four real MFMAs into
  * a real and an imaginary accumulator: re += yr xr - yi xi, im += yr xi +
- * yi xr.  The pair factor multiplies X in registers.  The FP32 MFMA leaves
- * row 4 (lane / 16) + r, column lane % 16 in register r (checked on the
- * device), so with Y as the A operand P runs along the lanes.
+ * yi xr.  The pair factor multiplies X in registers.  With Y as the A
+ * operand P runs along the lanes: the FP32 MFMA leaves row 4 (lane / 16) +
+ * r of column lane % 16 in register r, the FP64 one row 4 r + lane / 16
+ * (both checked on the device).
  */
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-template <typename T> struct m32 { static constexpr int W = 1; };
-template <> struct m32<cx<float>> { static constexpr int W = 2; };
+/* element -> real type R and components C */
+template <typename T> struct mdw { typedef float R; static constexpr int C = 1; };
+template <> struct mdw<cx<float>> { typedef float R; static constexpr int C = 2; };
+template <> struct mdw<cx<double>> { typedef double R; static constexpr int C = 2; };
 
-#define M32_BK 16
+__device__ __forceinline__ f4
+mfma16(float a, float b, f4 c)
+{
+	return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ d4
+mfma16(double a, double b, d4 c)
+{
+	return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
-template <typename T, int BQ>
-__global__ void __launch_bounds__(GEMM_WG, 4)
-k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
+template <typename T, int BQ, int M32_BK>
+__global__ void __launch_bounds__(GEMM_WG,
+    sizeof(typename mdw<T>::R) == 8 ? 2 : 4)
+k_gemm_mfma_dw(const gemm_tile *__restrict__ tiles,
     const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
     const gemm_scalars *__restrict__ scp)
 {
-	constexpr int W = m32<T>::W;		/* floats per element */
-	constexpr bool CX = W == 2;
+	typedef typename mdw<T>::R R;
+	typedef R acc_t __attribute__((ext_vector_type(4)));
+	constexpr int NC = mdw<T>::C;		/* components per element */
+	constexpr bool CX = NC == 2;
+	constexpr int W = NC * (int)sizeof(R) / 4;	/* words per element */
 	constexpr int NT = BQ / 32;		/* 16-column blocks per wave */
 	constexpr int XW = 64 * W, YW = BQ * W;	/* words of one column */
 	constexpr int SX = M32_BK * XW, SY = M32_BK * YW;	/* of a stage */
 	static_assert(SX % 256 == 0 && SY % 256 == 0, "DMAs per wave");
-	__shared__ float st[2][SX + SY];	/* [stage][X, Y][k][row][re, im] */
+	__shared__ __align__(16) float st[2][SX + SY];	/* [stage][X, Y][k][row] */
 
 	const gemm_tile tl = tiles[blockIdx.x];
 	if (tl.cblock == 0xffffffffu)
@@ -562,14 +579,14 @@ k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
 	const uint32_t pend = cb.pair_end;
 	const uint32_t pbeg = (sc.flags & 1) ? pend : cb.pair_begin;
 
-	f4 acc[2][NT][W];
+	acc_t acc[2][NT][NC];
 #pragma unroll
 	for (int i = 0; i < 2; i++)
 #pragma unroll
 		for (int j = 0; j < NT; j++)
 #pragma unroll
-			for (int c = 0; c < W; c++)
-				acc[i][j][c] = f4{0, 0, 0, 0};
+			for (int c = 0; c < NC; c++)
+				acc[i][j][c] = acc_t{0, 0, 0, 0};
 
 	/* step at column k0 of pair px into stage nx (X words, then Y words
 	 * at SX): DMA d of an operand moves its words [64 d, 64 d + 64); wave
@@ -626,50 +643,47 @@ k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
 			advance(fip, fk, fpr);
 		}
 		if (live) {
-			const float ar = (float)(sc.alpha_re * pr.a_re -
+			const R ar = (R)(sc.alpha_re * pr.a_re -
 			    sc.alpha_im * pr.a_im);
-			const float ai = (float)(sc.alpha_re * pr.a_im +
+			const R ai = (R)(sc.alpha_re * pr.a_im +
 			    sc.alpha_im * pr.a_re);
 #pragma unroll
 			for (int kk = 0; kk < M32_BK; kk += 4) {
 				const int kr = kk + (lane >> 4), c16 = lane & 15;
-				float xa[2][W], yb[NT][W];
+				R xa[2][NC], yb[NT][NC];
 #pragma unroll
 				for (int i = 0; i < 2; i++) {
-					const float *x = cur + kr * XW +
-					    (wp + i * 16 + c16) * W;
+					const R *x = (const R *)(cur + kr * XW +
+					    (wp + i * 16 + c16) * W);
 					if (CX) {
-						const float xr = x[0], xi = x[W - 1];
+						const R xr = x[0], xi = x[NC - 1];
 						xa[i][0] = xr * ar - xi * ai;
-						xa[i][W - 1] = xr * ai + xi * ar;
+						xa[i][NC - 1] = xr * ai + xi * ar;
 					} else {
 						xa[i][0] = x[0] * ar;
 					}
 				}
 #pragma unroll
 				for (int j = 0; j < NT; j++) {
-					const float *y = cur + SX + kr * YW +
-					    (wq + j * 16 + c16) * W;
+					const R *y = (const R *)(cur + SX + kr * YW +
+					    (wq + j * 16 + c16) * W);
 #pragma unroll
-					for (int c = 0; c < W; c++)
+					for (int c = 0; c < NC; c++)
 						yb[j][c] = y[c];
 				}
 #pragma unroll
 				for (int i = 0; i < 2; i++)
 #pragma unroll
 					for (int j = 0; j < NT; j++) {
-						acc[i][j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-						    yb[j][0], xa[i][0], acc[i][j][0], 0, 0, 0);
+						acc[i][j][0] = mfma16(yb[j][0], xa[i][0],
+						    acc[i][j][0]);
 						if (CX) {
-							acc[i][j][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-							    -yb[j][W - 1], xa[i][W - 1],
-							    acc[i][j][0], 0, 0, 0);
-							acc[i][j][W - 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-							    yb[j][0], xa[i][W - 1],
-							    acc[i][j][W - 1], 0, 0, 0);
-							acc[i][j][W - 1] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-							    yb[j][W - 1], xa[i][0],
-							    acc[i][j][W - 1], 0, 0, 0);
+							acc[i][j][0] = mfma16(-yb[j][NC - 1],
+							    xa[i][NC - 1], acc[i][j][0]);
+							acc[i][j][NC - 1] = mfma16(yb[j][0],
+							    xa[i][NC - 1], acc[i][j][NC - 1]);
+							acc[i][j][NC - 1] = mfma16(yb[j][NC - 1],
+							    xa[i][0], acc[i][j][NC - 1]);
 						}
 					}
 			}
@@ -699,12 +713,12 @@ k_gemm_mfma32(const gemm_tile *__restrict__ tiles,
 #pragma unroll
 			for (int r = 0; r < 4; r++) {
 				const uint32_t p = tl.p0 + wp + i * 16 + (lane & 15);
-				const uint32_t q = tl.q0 + wq + j * 16 +
-				    4 * (lane >> 4) + r;
+				const uint32_t q = tl.q0 + wq + j * 16 + (sizeof(R) == 4 ?
+				    4 * (lane >> 4) + r : 4 * r + (lane >> 4));
 				if (p < cb.p && q < cb.q) {
 					const size_t o = (size_t)p + (size_t)q * cb.ldc;
 					const T a = O::mk(acc[i][j][0][r],
-					    acc[i][j][W - 1][r]);
+					    acc[i][j][NC - 1][r]);
 					T v;
 					if (bz)
 						v = none ? O::zero() : a;
@@ -759,7 +773,7 @@ tile_candidates(const contract_plan *pl, std::vector<std::pair<uint32_t,
 			out.push_back({64, 128});
 			out.push_back({64, 64});
 		}
-	} else if (pl->f32_mfma && pl->type == XM_SCALAR_FLOAT) {
+	} else if (pl->mfma_dw && pl->type == XM_SCALAR_FLOAT) {
 		/* (complex64 at 64 x 128 would hold 259 registers: 1 wave) */
 		const char *e = getenv("XM_GEMM_TILE");
 		if (e && strcmp(e, "64x64") == 0)
@@ -873,13 +887,14 @@ f64_dma_eligible(const contract_plan *pl)
 	return true;
 }
 
-/* Every X and Y panel P-contiguous in an FP32 or complex64 plan: what
- * k_gemm_mfma32 reads.  XM_GEMM_SIMPLE=1 keeps the FMA kernel. */
+/* Every X and Y panel P-contiguous in an FP32, complex64 or complex128
+ * plan: what
+ * k_gemm_mfma_dw reads.  XM_GEMM_SIMPLE=1 keeps the FMA kernel. */
 static bool
-f32_mfma_eligible(const contract_plan *pl)
+mfma_dw_eligible(const contract_plan *pl)
 {
-	if ((pl->type != XM_SCALAR_FLOAT && pl->type != XM_SCALAR_FLOAT_COMPLEX) ||
-	    use_simple())
+	if ((pl->type != XM_SCALAR_FLOAT && pl->type != XM_SCALAR_FLOAT_COMPLEX &&
+	    pl->type != XM_SCALAR_DOUBLE_COMPLEX) || use_simple())
 		return false;
 	for (const gemm_pair &g : pl->pairs)
 		if (g.xmode != PANEL_PCONTIG || g.ymode != PANEL_PCONTIG)
@@ -891,7 +906,7 @@ void
 build_tiles(contract_plan *pl)
 {
 	pl->f64_dma = f64_dma_eligible(pl);
-	pl->f32_mfma = f32_mfma_eligible(pl);
+	pl->mfma_dw = mfma_dw_eligible(pl);
 	std::vector<std::pair<uint32_t, uint32_t>> cand;
 	tile_candidates(pl, cand);
 	pl->ncand = (int)std::min<size_t>(cand.size(), MAX_TILE_CANDIDATES);
@@ -973,27 +988,31 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_FLOAT:
-		if (pl->f32_mfma && pl->cand_bq[pl->tile_active] == 128)
-			k_gemm_mfma32<float, 128><<<g, b, 0, s>>>(pl->d_tiles,
+		if (pl->mfma_dw && pl->cand_bq[pl->tile_active] == 128)
+			k_gemm_mfma_dw<float, 128, 16><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
-		else if (pl->f32_mfma)
-			k_gemm_mfma32<float, 64><<<g, b, 0, s>>>(pl->d_tiles,
+		else if (pl->mfma_dw)
+			k_gemm_mfma_dw<float, 64, 16><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		else
 			k_gemm_simple<float><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_FLOAT_COMPLEX:
-		if (pl->f32_mfma)
-			k_gemm_mfma32<cx<float>, 64><<<g, b, 0, s>>>(pl->d_tiles,
+		if (pl->mfma_dw)
+			k_gemm_mfma_dw<cx<float>, 64, 16><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		else
 			k_gemm_simple<cx<float>><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_DOUBLE_COMPLEX:
-		k_gemm_simple<cx<double>><<<g, b, 0, s>>>(pl->d_tiles,
-		    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		if (pl->mfma_dw)
+			k_gemm_mfma_dw<cx<double>, 64, 8><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		else
+			k_gemm_simple<cx<double>><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	}
 	hipError_t e = hipGetLastError();
