@@ -216,17 +216,22 @@ fdiv_q(uint32_t n, const fastdiv &f)
  * workgroup, and for absent sources.
  *
  * Tiled form (elem_tile_dev, kernel k_permute): destination dim 0 (extent
- * A) against the source's fastest dim t (extent B).  A workgroup moves NS
- * slabs of one TA x TB tile, the slabs being consecutive values of the
- * remaining ("batch") dims: it reads them in source order (runs of TB
- * along dim t) into LDS, then writes them in destination order (runs of TA
- * along dim 0).  Without it, the lines of a far-transposed source are
- * fetched once per element by workgroups spread over the XCDs' L2s.
+ * A) against the source's fastest dim t (extent B).  A unit is NS slabs of
+ * one TA x TB tile, the slabs being consecutive values of the remaining
+ * ("batch") dims: it is read in source order (runs of TB along dim t) into
+ * LDS, then written in destination order (runs of TA along dim 0).  A
+ * workgroup moves R consecutive units, the next unit's loads in flight in
+ * registers while the current one is written out.  Without the tile, the
+ * lines of a far-transposed source are fetched once per element by
+ * workgroups spread over the XCDs' L2s.
  */
 #define TILE_MAX 32			/* TA, TB <= 32 */
 #define TILE_BYTES (64 * 33 * 8)	/* tile storage, rows padded to TB + 1 */
 #define TILE_NS_MAX 128
-#define TILE_LDS (TILE_BYTES + 2 * TILE_NS_MAX * 8)
+#define TILE_PER 9			/* >= TILE_BYTES / 8 / WG elements */
+#define TILE_R 8			/* units per workgroup, at most */
+/* tile | two (source, destination) slab-base buffers */
+#define TILE_LDS (TILE_BYTES + 4 * TILE_NS_MAX * 8)
 
 struct elem_job_dev {
 	uint64_t dst, src, n, first_chunk;
@@ -241,7 +246,8 @@ struct elem_tile_dev {
 	double f_re, f_im;
 	uint32_t nbd;			/* batch dims */
 	uint32_t A, B, TA, TB, NS, nta, ntb;
-	uint64_t nbatch;
+	uint32_t R, pad;		/* units per workgroup */
+	uint64_t nbatch, nunits;
 	uint64_t sa, sb, db;		/* source strides of dims 0 and t;
 					 * destination stride of t */
 	fastdiv f_tatb, f_tb, f_ta;
@@ -336,30 +342,37 @@ k_elem(const elem_job_dev *jobs, uint32_t njobs, double al_re, double al_im,
 	}
 }
 
-template <typename D, typename S, int OP>
-__global__ void __launch_bounds__(WG)
-k_permute(const elem_tile_dev *jobs, uint32_t njobs, double al_re,
-    double al_im, int alpha_zero)
+/* Unit u of a tiled job: tile origin, extents and first slab. */
+struct tile_unit {
+	uint32_t a0, b0, ta, tb, ns;
+	uint64_t s0;
+};
+
+__device__ __forceinline__ tile_unit
+unit_of(const elem_tile_dev &jb, uint64_t u)
 {
-	extern __shared__ __align__(16) char tile_lds[];
-	S *tile = (S *)tile_lds;
-	uint64_t *sbase = (uint64_t *)(tile_lds + TILE_BYTES);
-	uint64_t *dbase = sbase + TILE_NS_MAX;
-	const uint64_t chunk = blockIdx.x;
-	const uint32_t j = find_job(jobs, njobs, chunk);
-	__shared__ elem_tile_dev jb;
-	load_job(&jb, jobs + j);
-	const uint64_t local = chunk - jb.first_chunk;
-	const uint32_t ia = (uint32_t)(local % jb.nta);
-	const uint64_t r1 = local / jb.nta;
+	tile_unit t;
+	const uint32_t ia = (uint32_t)(u % jb.nta);
+	const uint64_t r1 = u / jb.nta;
 	const uint32_t ib = (uint32_t)(r1 % jb.ntb);
-	const uint64_t s0 = r1 / jb.ntb * jb.NS;
-	const uint32_t a0 = ia * jb.TA, b0 = ib * jb.TB;
-	const uint32_t ta = min(jb.TA, jb.A - a0), tb = min(jb.TB, jb.B - b0);
-	const uint32_t ns = (uint32_t)min((uint64_t)jb.NS, jb.nbatch - s0);
-	for (uint32_t s = threadIdx.x; s < ns; s += WG) {
-		uint32_t q = (uint32_t)(s0 + s);
-		uint64_t so = a0 * jb.sa + b0 * jb.sb, dof = a0 + b0 * jb.db;
+	t.s0 = r1 / jb.ntb * jb.NS;
+	t.a0 = ia * jb.TA;
+	t.b0 = ib * jb.TB;
+	t.ta = min(jb.TA, jb.A - t.a0);
+	t.tb = min(jb.TB, jb.B - t.b0);
+	t.ns = (uint32_t)min((uint64_t)jb.NS, jb.nbatch - t.s0);
+	return t;
+}
+
+/* Source and destination offsets of the unit's slabs into sb / db. */
+__device__ __forceinline__ void
+unit_bases(const elem_tile_dev &jb, const tile_unit &t, uint64_t *sb,
+    uint64_t *db)
+{
+	for (uint32_t s = threadIdx.x; s < t.ns; s += WG) {
+		uint32_t q = (uint32_t)(t.s0 + s);
+		uint64_t so = t.a0 * jb.sa + t.b0 * jb.sb;
+		uint64_t dof = t.a0 + t.b0 * jb.db;
 		for (uint32_t k = 0; k < jb.nbd; k++) {
 			const uint32_t qq = fdiv_q(q, jb.bdiv[k]);
 			const uint32_t r = q - qq * jb.bdiv[k].d;
@@ -367,43 +380,96 @@ k_permute(const elem_tile_dev *jobs, uint32_t njobs, double al_re,
 			dof += r * jb.bd[k];
 			q = qq;
 		}
-		sbase[s] = so;
-		dbase[s] = dof;
+		sb[s] = so;
+		db[s] = dof;
 	}
-	__syncthreads();
+}
+
+template <typename D, typename S, int OP>
+__global__ void __launch_bounds__(WG)
+k_permute(const elem_tile_dev *jobs, uint32_t njobs, double al_re,
+    double al_im, int alpha_zero)
+{
+	extern __shared__ __align__(16) char tile_lds[];
+	S *tile = (S *)tile_lds;
+	uint64_t *bases = (uint64_t *)(tile_lds + TILE_BYTES);
+	const uint64_t chunk = blockIdx.x;
+	const uint32_t j = find_job(jobs, njobs, chunk);
+	__shared__ elem_tile_dev jb;
+	load_job(&jb, jobs + j);
+	const uint64_t u0 = (chunk - jb.first_chunk) * jb.R;
+	const uint32_t cnt = (uint32_t)min((uint64_t)jb.R, jb.nunits - u0);
 	D *dst = (D *)(uintptr_t)jb.dst;
 	const S *src = (const S *)(uintptr_t)jb.src;
 	const S f = mk<S>(jb.f_re, jb.f_im);
 	const D alpha = mk<D>(al_re, al_im);
 	const uint32_t TA = jb.TA, TB = jb.TB, TATB = TA * TB;
 	const uint32_t tot = jb.NS * TATB, row = TB + 1;
-	/* source order (b fastest) into the tile */
-	for (uint32_t e = threadIdx.x; e < tot; e += WG) {
-		const uint32_t s = fdiv_q(e, jb.f_tatb);
-		const uint32_t r = e - s * TATB;
-		const uint32_t a = fdiv_q(r, jb.f_tb);
-		const uint32_t b = r - a * TB;
-		S v = mk<S>(0.0, 0.0);
-		if (s < ns && a < ta && b < tb)
-			v = src[sbase[s] + a * jb.sa + b * jb.sb];
-		tile[(s * TA + a) * row + b] = v;
-	}
-	__syncthreads();
-	/* destination order (a fastest) out of the tile */
-	for (uint32_t e = threadIdx.x; e < tot; e += WG) {
-		const uint32_t s = fdiv_q(e, jb.f_tatb);
-		const uint32_t r = e - s * TATB;
-		const uint32_t b = fdiv_q(r, jb.f_ta);
-		const uint32_t a = r - b * TA;
-		if (s < ns && a < ta && b < tb) {
-			const uint64_t i = dbase[s] + a + b * jb.db;
-			D d = mk<D>(0.0, 0.0);
-			if (reads_dst<OP>::v)
-				d = dst[i];
-			dst[i] = elem_op<D, S, OP>(d,
-			    tile[(s * TA + a) * row + b], true, f, alpha,
-			    alpha_zero);
+
+	/* loads of unit t (slab bases in sb) into v, source order */
+	S v[TILE_PER];
+	auto load = [&](const tile_unit &t, const uint64_t *sb) {
+#pragma unroll
+		for (int u = 0; u < TILE_PER; u++) {
+			const uint32_t e = threadIdx.x + u * WG;
+			const uint32_t s = fdiv_q(e, jb.f_tatb);
+			const uint32_t r = e - s * TATB;
+			const uint32_t a = fdiv_q(r, jb.f_tb);
+			const uint32_t b = r - a * TB;
+			v[u] = mk<S>(0.0, 0.0);
+			if (e < tot && s < t.ns && a < t.ta && b < t.tb)
+				v[u] = src[sb[s] + a * jb.sa + b * jb.sb];
 		}
+	};
+	auto to_tile = [&]() {
+#pragma unroll
+		for (int u = 0; u < TILE_PER; u++) {
+			const uint32_t e = threadIdx.x + u * WG;
+			const uint32_t s = fdiv_q(e, jb.f_tatb);
+			const uint32_t r = e - s * TATB;
+			const uint32_t a = fdiv_q(r, jb.f_tb);
+			const uint32_t b = r - a * TB;
+			if (e < tot)
+				tile[(s * TA + a) * row + b] = v[u];
+		}
+	};
+
+	tile_unit cur = unit_of(jb, u0);
+	unit_bases(jb, cur, bases, bases + TILE_NS_MAX);
+	__syncthreads();
+	load(cur, bases);
+	for (uint32_t k = 0; k < cnt; k++) {
+		uint64_t *sb = bases + (k & 1) * 2 * TILE_NS_MAX;
+		uint64_t *db = sb + TILE_NS_MAX;
+		uint64_t *nsb = bases + ((k + 1) & 1) * 2 * TILE_NS_MAX;
+		to_tile();
+		tile_unit nxt = cur;
+		const bool more = k + 1 < cnt;
+		if (more) {
+			nxt = unit_of(jb, u0 + k + 1);
+			unit_bases(jb, nxt, nsb, nsb + TILE_NS_MAX);
+		}
+		__syncthreads();
+		if (more)
+			load(nxt, nsb);
+		/* destination order (a fastest) out of the tile */
+		for (uint32_t e = threadIdx.x; e < tot; e += WG) {
+			const uint32_t s = fdiv_q(e, jb.f_tatb);
+			const uint32_t r = e - s * TATB;
+			const uint32_t b = fdiv_q(r, jb.f_ta);
+			const uint32_t a = r - b * TA;
+			if (s < cur.ns && a < cur.ta && b < cur.tb) {
+				const uint64_t i = db[s] + a + b * jb.db;
+				D d = mk<D>(0.0, 0.0);
+				if (reads_dst<OP>::v)
+					d = dst[i];
+				dst[i] = elem_op<D, S, OP>(d,
+				    tile[(s * TA + a) * row + b], true, f, alpha,
+				    alpha_zero);
+			}
+		}
+		__syncthreads();
+		cur = nxt;
 	}
 }
 
@@ -569,7 +635,11 @@ plan_tiled(elem_tile_dev &b, const uint32_t *dims, const uint64_t *str,
 	}
 	for (uint32_t k = b.nbd; k < XM_ELEM_MAXDIM; k++)
 		b.bdiv[k] = make_fdiv(1);
-	uint64_t ns = (TILE_BYTES / esz) / ((uint64_t)b.TA * (b.TB + 1));
+	/* slabs per unit: within the tile storage and the TILE_PER
+	 * elements a thread holds in flight */
+	uint64_t ns = std::min<uint64_t>(
+	    (TILE_BYTES / esz) / ((uint64_t)b.TA * (b.TB + 1)),
+	    (uint64_t)TILE_PER * WG / ((uint64_t)b.TA * b.TB));
 	ns = std::min<uint64_t>(std::max<uint64_t>(ns, 1), TILE_NS_MAX);
 	b.NS = (uint32_t)std::min<uint64_t>(ns, b.nbatch);
 	b.f_tatb = make_fdiv(b.TA * b.TB);
@@ -620,9 +690,9 @@ to_dev(const elem_job *jobs, size_t n, size_t esz, bool tiling,
 			tb.src = a.src;
 			tb.f_re = a.f_re;
 			tb.f_im = a.f_im;
-			tb.first_chunk = ct;
-			ct += (uint64_t)tb.nta * tb.ntb *
+			tb.nunits = (uint64_t)tb.nta * tb.ntb *
 			    ((tb.nbatch + tb.NS - 1) / tb.NS);
+			ct += tb.nunits;
 			t.push_back(tb);
 			continue;
 		}
@@ -643,10 +713,20 @@ to_dev(const elem_job *jobs, size_t n, size_t esz, bool tiling,
 		cg += (a.n + CHUNK - 1) / CHUNK;
 		g.push_back(b);
 	}
+	/* units per workgroup: as many as keep 4096 workgroups (16 per CU)
+	 * in the launch, at most TILE_R */
+	const uint32_t R = (uint32_t)std::max<uint64_t>(1,
+	    std::min<uint64_t>(TILE_R, ct / 4096));
+	uint64_t cw = 0;
+	for (auto &tb : t) {
+		tb.R = R;
+		tb.first_chunk = cw;
+		cw += (tb.nunits + R - 1) / R;
+	}
 	l->ngather = g.size();
 	l->gchunks = cg;
 	l->ntiled = t.size();
-	l->tchunks = ct;
+	l->tchunks = cw;
 	l->toff = (g.size() * sizeof(elem_job_dev) + 255) / 256 * 256;
 	std::vector<char> out(l->toff + t.size() * sizeof(elem_tile_dev), 0);
 	if (!g.empty())
