@@ -492,6 +492,39 @@ def cpu_permute(sp):
     return res
 
 
+def backward_mode(lib, args):
+    """xm_contract_backward of the o=64 v=384 ladder: dt and dv from dc (c's
+    structure, U[-1,1]); FP64 and split (5 slices); TFLOP/s of the two
+    adjoint contractions' F together, wall time per call."""
+    from libxm_amd import xm, workloads
+    w = workloads.ladder_d2h(args.o, args.v, args.seed)
+    alloc, t, v, c = build_workload(lib, w, args.seed)
+    dc = w.c.build(lib, alloc)
+    fill_tensor(lib, w, dc, args.seed * 16 + 5)
+    dt, dv = w.t.build(lib, alloc), w.v.build(lib, alloc)
+    out = {"mode": "backward", "workload": w.name, "forms": {}}
+    for name, prec, ns in (("fp64", xm.PRECISION_NATIVE, None),
+                           ("split5", xm.PRECISION_SPLIT_BF16, 5)):
+        lib.set_precision(prec, ns)
+        call = lambda: lib.contract_backward(w.alpha, t, v, dc, dt, dv,
+                                             *w.idx)
+        for _ in range(max(args.warmup, 3)):    # plans and tile tuning
+            call()
+        ms = timed(lib, call, max(args.steps, 3))
+        st = lib.stats()
+        out["forms"][name] = {"ms_per_call": ms, "flops": st["flops"],
+                              "tflops": st["flops"] / ms / 1e9,
+                              "gemm_ms": st["gemm_ms"],
+                              "prep_ms": st["prep_ms"],
+                              "post_ms": st["post_ms"]}
+        log("backward %s: %.3f ms/call %.2f TFLOP/s (GEMM %.3f, permutes "
+            "and slices %.3f + %.3f ms)" % (name, ms, st["flops"] / ms / 1e9,
+                                          st["gemm_ms"], st["prep_ms"],
+                                          st["post_ms"]))
+    lib.set_precision(xm.PRECISION_NATIVE)
+    print(json.dumps(out), flush=True)
+
+
 def overflow_mode(lib, args):
     """The o=64 v=384 ladder with t and c in HBM and v (7.4 GB) in pinned
     host memory (arena capped at 1 GiB before v is built): one contraction
@@ -542,7 +575,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--mode", choices=["contract", "copy", "ring", "precision",
-                                       "permute", "overflow"],
+                                       "permute", "overflow", "backward"],
                     default="contract",
                     help="copy: xm_copy of a 4 GiB FP64 tensor, a plain "
                          "copy of known size (HBM counter calibration); "
@@ -551,7 +584,8 @@ def main():
                          "ladder over six decades in FP64, split and FP32; "
                          "permute: xm_copy under six permutations of 4-D "
                          "tensors with blocks of 32, 16 and 4-28; overflow: "
-                         "the ladder with v in pinned host memory")
+                         "the ladder with v in pinned host memory; backward: "
+                         "xm_contract_backward of the ladder")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build runs on a single "
@@ -570,6 +604,8 @@ def main():
         return permute_mode(lib, args)
     if args.mode == "overflow":
         return overflow_mode(lib, args)
+    if args.mode == "backward":
+        return backward_mode(lib, args)
     w = workloads.ladder_d2h(args.o, args.v, args.seed)
     log("workload", json.dumps(w.describe()))
     tb = time.perf_counter()

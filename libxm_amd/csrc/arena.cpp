@@ -228,11 +228,14 @@ xm_allocator_write(xm_allocator_t *a, uint64_t p, const void *mem,
 	HIP_OK("xm_allocator_write", hipStreamSynchronize(xm_stream()));
 }
 
+void bwd_cache_drop_alloc(xm_allocator_t *a);
+
 XM_API void
 xm_allocator_destroy(xm_allocator_t *a)
 {
 	if (a == nullptr)
 		return;
+	bwd_cache_drop_alloc(a);
 	if (!a->slabs.empty())
 		HIP_OK("xm_allocator_destroy",
 		    hipStreamSynchronize(xm_stream()));

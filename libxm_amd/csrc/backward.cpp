@@ -23,8 +23,11 @@
  */
 #include "contract_plan.h"
 
+#include <algorithm>
 #include <chrono>
+#include <list>
 #include <map>
+#include <mutex>
 
 static const char *FN = "xm_contract_backward";
 
@@ -40,19 +43,18 @@ canonical_view(const xm_tensor_t *dc)
 	return v;
 }
 
-/* One gradient: g = every nonzero block of x made canonical; canonical
- * blocks share dx's storage. */
-static void
-gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
-    const xm_tensor_t *p, const xm_tensor_t *q, const char *ip,
-    const char *iq, const char *ix, bool x_first, xm_contract_stats_t *acc)
+/* The gradient tensor of x: every nonzero block of x made canonical;
+ * canonical blocks share dx's storage, derivative ones get scratch
+ * (returned in scratch). */
+static xm_tensor_t *
+gradient_tensor(const xm_tensor_t *x, xm_tensor_t *dx,
+    std::vector<uint64_t> &scratch)
 {
 	if (dx->type != x->type)
 		XM_FATAL(FN, "gradient tensors must have the tensor's scalar type");
 	if (!xm_block_space_eq(dx->bs, x->bs))
 		XM_FATAL(FN, "gradient tensors must have the tensor's block space");
 	xm_tensor_t *g = xm_tensor_create(x->bs, x->type, x->alloc);
-	std::vector<uint64_t> scratch;
 	for (size_t f = 0; f < x->blocks.size(); f++) {
 		const xm_block &bx = x->blocks[f];
 		if (bx.type == XM_BLOCK_TYPE_ZERO)
@@ -69,6 +71,93 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 			scratch.push_back(g->blocks[f].data);
 		}
 	}
+	return g;
+}
+
+/*
+ * The auxiliary tensors of one (a, b, dc, da, db) combination: dc's
+ * canonical view and the two gradient tensors, kept across calls while
+ * none of the five changes structure, so repeated calls find their
+ * contraction plans cached (a new tensor would be a new plan key).
+ */
+struct bwd_aux {
+	uint64_t id[5], ver[5];
+	xm_tensor_t *dcv = nullptr, *ga = nullptr, *gb = nullptr;
+	std::vector<uint64_t> sa, sb;		/* scratch of ga, gb */
+	xm_allocator_t *aa = nullptr, *ab = nullptr;
+};
+
+#define BWD_CACHE_MAX 4
+static std::list<bwd_aux> bwd_cache;
+static std::mutex bwd_mu;
+
+static void
+aux_release(bwd_aux &x)
+{
+	for (uint64_t p : x.sa)
+		xm_allocator_deallocate(x.aa, p);
+	for (uint64_t p : x.sb)
+		xm_allocator_deallocate(x.ab, p);
+	/* metadata only (plans on them are dropped); storage released above
+	 * or owned by dc, da, db */
+	for (xm_tensor_t *t : {x.ga, x.gb, x.dcv})
+		if (t)
+			xm_tensor_free(t);
+}
+
+/* Forget the auxiliary tensors that involve tensor `id` (being freed). */
+void
+bwd_cache_drop(uint64_t id)
+{
+	std::list<bwd_aux> gone;
+	{
+		std::lock_guard<std::mutex> g(bwd_mu);
+		for (auto it = bwd_cache.begin(); it != bwd_cache.end();) {
+			bool hit = false;
+			for (uint64_t k : it->id)
+				hit |= k == id;
+			if (hit) {
+				auto nx = std::next(it);
+				gone.splice(gone.end(), bwd_cache, it);
+				it = nx;
+			} else {
+				++it;
+			}
+		}
+	}
+	for (auto &x : gone)
+		aux_release(x);
+}
+
+/* Forget the auxiliary tensors whose scratch lives in allocator `a` (being
+ * destroyed: its storage goes with it). */
+void
+bwd_cache_drop_alloc(xm_allocator_t *a)
+{
+	std::list<bwd_aux> gone;
+	{
+		std::lock_guard<std::mutex> g(bwd_mu);
+		for (auto it = bwd_cache.begin(); it != bwd_cache.end();) {
+			auto nx = std::next(it);
+			if (it->aa == a || it->ab == a || (it->dcv &&
+			    it->dcv->alloc == a))
+				gone.splice(gone.end(), bwd_cache, it);
+			it = nx;
+		}
+	}
+	for (auto &x : gone) {
+		x.sa.clear();
+		x.sb.clear();
+		aux_release(x);
+	}
+}
+
+/* One gradient: g from gradient_tensor. */
+static void
+gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
+    xm_tensor_t *g, const xm_tensor_t *p, const xm_tensor_t *q,
+    const char *ip, const char *iq, const char *ix, xm_contract_stats_t *acc)
+{
 	xm_contract(alpha, p, q, 0, g, ip, iq, ix);
 	xm_contract_stats_t st;
 	xm_contract_get_stats(&st);
@@ -90,7 +179,7 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 	acc->permute_gather += st.permute_gather;
 	acc->precision = st.precision;
 	acc->nslices = st.nslices;
-	(void)x_first;
+	acc->plan_cached = st.plan_cached;
 
 	/* fold derivative blocks into their sources:
 	 * dx[S](permute(e, p)) += s * g[X](e) */
@@ -133,9 +222,6 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 		    0, s);
 		HIP_OK(FN, hipStreamSynchronize(s));
 	}
-	for (uint64_t ptr : scratch)
-		xm_allocator_deallocate(x->alloc, ptr);
-	xm_tensor_free(g);	/* metadata only; storage released above */
 }
 
 XM_API void
@@ -151,12 +237,51 @@ xm_contract_backward(xm_scalar_t alpha, const xm_tensor_t *a,
 		XM_FATAL(FN, "tensors must use same allocator");
 	xm_contract_stats_t acc;
 	memset(&acc, 0, sizeof acc);
-	xm_tensor_t *dcv = canonical_view(dc);
+	const xm_tensor_t *ts[5] = {a, b, dc, da, db};
+	bwd_aux cur;
+	for (int k = 0; k < 5; k++) {
+		cur.id[k] = ts[k] ? ts[k]->id : 0;
+		cur.ver[k] = ts[k] ? ts[k]->version : 0;
+	}
+	bwd_aux *aux = nullptr;
+	{
+		std::lock_guard<std::mutex> g(bwd_mu);
+		for (auto it = bwd_cache.begin(); it != bwd_cache.end(); ++it)
+			if (std::equal(cur.id, cur.id + 5, it->id) &&
+			    std::equal(cur.ver, cur.ver + 5, it->ver)) {
+				bwd_cache.splice(bwd_cache.begin(), bwd_cache, it);
+				aux = &bwd_cache.front();
+				break;
+			}
+	}
+	if (aux == nullptr) {
+		cur.dcv = canonical_view(dc);
+		if (da) {
+			cur.ga = gradient_tensor(a, da, cur.sa);
+			cur.aa = a->alloc;
+		}
+		if (db) {
+			cur.gb = gradient_tensor(b, db, cur.sb);
+			cur.ab = b->alloc;
+		}
+		std::list<bwd_aux> gone;
+		{
+			std::lock_guard<std::mutex> g(bwd_mu);
+			bwd_cache.push_front(std::move(cur));
+			aux = &bwd_cache.front();
+			while (bwd_cache.size() > BWD_CACHE_MAX)
+				gone.splice(gone.end(), bwd_cache,
+				    std::prev(bwd_cache.end()));
+		}
+		for (auto &x : gone)
+			aux_release(x);
+	}
 	if (da)
-		gradient(alpha, a, da, dcv, b, idxc, idxb, idxa, true, &acc);
+		gradient(alpha, a, da, aux->ga, aux->dcv, b, idxc, idxb, idxa,
+		    &acc);
 	if (db)
-		gradient(alpha, b, db, a, dcv, idxa, idxc, idxb, false, &acc);
-	xm_tensor_free(dcv);
+		gradient(alpha, b, db, aux->gb, a, aux->dcv, idxa, idxc, idxb,
+		    &acc);
 	auto t1 = std::chrono::steady_clock::now();
 	acc.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
 	contract_set_stats(acc);

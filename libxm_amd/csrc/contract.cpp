@@ -320,6 +320,25 @@ struct panel_key {
 	}
 };
 
+/* Whether a plan's operand panels are laid out P-contiguous, for the
+ * DMA-staged GEMM kernels that read only those: FP32 and complex plans,
+ * and native FP64 plans whose C blocks all have even extents (the FP64
+ * kernel's row-pair DMAs need them; with an odd one the plan runs the
+ * register-staged kernel, which reads K-contiguous panels as well and ran
+ * 3-4% faster on them).  XM_PANEL_PCONTIG=0: only where the source's
+ * stride-1 index is free. */
+static bool
+pcontig_panels(const contract_plan *pl)
+{
+	static int on = -1;
+	if (on < 0) {
+		const char *e = getenv("XM_PANEL_PCONTIG");
+		on = !(e && *e == '0');
+	}
+	return on && pl->precision == XM_PRECISION_NATIVE &&
+	    (pl->type != XM_SCALAR_DOUBLE || pl->even_extents);
+}
+
 struct plan_builder {
 	contract_plan *pl;
 	size_t esz;
@@ -345,11 +364,10 @@ struct plan_builder {
 		 * copy is unfolded into a panel */
 		const uint64_t slab = stage ? arena_host_slab(alloc, src) : 0;
 		const bool host = slab != 0;
-		/* FP32 and complex plans want P-contiguous panels: their MFMA
-		 * kernel reads only those */
-		const bool pfirst = pl->type == XM_SCALAR_FLOAT ||
-		    pl->type == XM_SCALAR_FLOAT_COMPLEX ||
-		    pl->type == XM_SCALAR_DOUBLE_COMPLEX;
+		/* plans on the staged MFMA kernels (native FP64, FP32,
+		 * complex) want P-contiguous panels: those kernels read only
+		 * them */
+		const bool pfirst = pcontig_panels(pl);
 		if (!host && direct_layout(ls, d, F, Kd, mode, ld) &&
 		    !(pfirst && *mode == PANEL_KCONTIG)) {
 			*ptr = src;
@@ -436,6 +454,13 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 	hi = std::min(hi, nlist);
 	lo = std::min(lo, hi);
 	pl->nblocks_c = hi - lo;
+	pl->even_extents = true;
+	for (size_t ic = lo; ic < hi; ic++) {
+		xm_dim_t dc = block_dims_flat(c, block_flat(c, list[ic]));
+		if (xm_dim_dot_mask(&dc, &mk.cidxc) % 2 ||
+		    xm_dim_dot_mask(&dc, &mk.aidxc) % 2)
+			pl->even_extents = false;
+	}
 
 	std::vector<block_pair> all;
 	std::vector<std::vector<uint32_t>> members;

@@ -1115,11 +1115,13 @@ xm_tensor_free_block_data(xm_tensor_t *t)
 
 void plan_cache_drop(uint64_t tensor_id);
 void op_cache_drop(uint64_t tensor_id);
+void bwd_cache_drop(uint64_t tensor_id);
 
 XM_API void
 xm_tensor_free(xm_tensor_t *t)
 {
 	if (t) {
+		bwd_cache_drop(t->id);
 		plan_cache_drop(t->id);
 		op_cache_drop(t->id);
 		xm_block_space_free(t->bs);
