@@ -80,11 +80,19 @@ gradient_tensor(const xm_tensor_t *x, xm_tensor_t *dx,
  * none of the five changes structure, so repeated calls find their
  * contraction plans cached (a new tensor would be a new plan key).
  */
+struct fold_wave {
+	void *dev;			/* packed jobs, device copy */
+	elem_launch l;
+};
+
 struct bwd_aux {
 	uint64_t id[5], ver[5];
 	xm_tensor_t *dcv = nullptr, *ga = nullptr, *gb = nullptr;
 	std::vector<uint64_t> sa, sb;		/* scratch of ga, gb */
 	xm_allocator_t *aa = nullptr, *ab = nullptr;
+	/* fold launches of ga, gb (built by their first call) */
+	std::vector<fold_wave> fa, fb;
+	bool fa_built = false, fb_built = false;
 };
 
 #define BWD_CACHE_MAX 4
@@ -94,6 +102,10 @@ static std::mutex bwd_mu;
 static void
 aux_release(bwd_aux &x)
 {
+	for (auto *w : {&x.fa, &x.fb})
+		for (fold_wave &f : *w)
+			if (f.dev)
+				HIP_OK(FN, hipFree(f.dev));
 	for (uint64_t p : x.sa)
 		xm_allocator_deallocate(x.aa, p);
 	for (uint64_t p : x.sb)
@@ -152,11 +164,13 @@ bwd_cache_drop_alloc(xm_allocator_t *a)
 	}
 }
 
-/* One gradient: g from gradient_tensor. */
+/* One gradient: g from gradient_tensor; fold: its fold launches, built
+ * on the first call (built). */
 static void
 gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
     xm_tensor_t *g, const xm_tensor_t *p, const xm_tensor_t *q,
-    const char *ip, const char *iq, const char *ix, xm_contract_stats_t *acc)
+    const char *ip, const char *iq, const char *ix, xm_contract_stats_t *acc,
+    std::vector<fold_wave> &fold, bool &built)
 {
 	xm_contract(alpha, p, q, 0, g, ip, iq, ix);
 	xm_contract_stats_t st;
@@ -182,7 +196,16 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 	acc->plan_cached = st.plan_cached;
 
 	/* fold derivative blocks into their sources:
-	 * dx[S](permute(e, p)) += s * g[X](e) */
+	 * dx[S](permute(e, p)) += s * g[X](e); one launch per wave, and a
+	 * wave never writes one source block twice */
+	hipStream_t s = xm_stream();
+	if (built) {
+		for (const fold_wave &f : fold)
+			launch_elem_packed(f.dev, f.l, ELEM_ACCUM, x->type,
+			    x->type, 0, s);
+		HIP_OK(FN, hipStreamSynchronize(s));
+		return;
+	}
 	std::map<size_t, int> waves_of;	/* source -> jobs so far */
 	std::vector<std::vector<elem_job>> waves;
 	for (size_t f = 0; f < x->blocks.size(); f++) {
@@ -216,12 +239,21 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 		if (j.n)
 			waves[w].push_back(j);
 	}
-	hipStream_t s = xm_stream();
 	for (auto &wv : waves) {
-		launch_elem(wv.data(), wv.size(), ELEM_ACCUM, x->type, x->type,
+		if (wv.empty())
+			continue;
+		fold_wave fw{nullptr, elem_launch()};
+		std::vector<char> pk = elem_jobs_pack(wv, elem_size(x->type),
+		    &fw.l);
+		HIP_OK(FN, hipMalloc(&fw.dev, pk.size()));
+		HIP_OK(FN, hipMemcpy(fw.dev, pk.data(), pk.size(),
+		    hipMemcpyHostToDevice));
+		fold.push_back(fw);
+		launch_elem_packed(fw.dev, fw.l, ELEM_ACCUM, x->type, x->type,
 		    0, s);
-		HIP_OK(FN, hipStreamSynchronize(s));
 	}
+	built = true;
+	HIP_OK(FN, hipStreamSynchronize(s));
 }
 
 XM_API void
@@ -278,10 +310,10 @@ xm_contract_backward(xm_scalar_t alpha, const xm_tensor_t *a,
 	}
 	if (da)
 		gradient(alpha, a, da, aux->ga, aux->dcv, b, idxc, idxb, idxa,
-		    &acc);
+		    &acc, aux->fa, aux->fa_built);
 	if (db)
 		gradient(alpha, b, db, aux->gb, a, aux->dcv, idxa, idxc, idxb,
-		    &acc);
+		    &acc, aux->fb, aux->fb_built);
 	auto t1 = std::chrono::steady_clock::now();
 	acc.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
 	contract_set_stats(acc);
