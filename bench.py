@@ -588,8 +588,9 @@ def main():
                          "xm_contract_backward of the ladder")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU; this build runs on a single "
-              "MI355X (--gpus 1).", file=sys.stderr)
+        print("bench.py measures one GPU: the library runs on a single "
+              "MI355X (--gpus 1); sharding is described, not built.",
+              file=sys.stderr)
         sys.exit(2)
 
     from libxm_amd import xm, workloads
