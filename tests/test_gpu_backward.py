@@ -180,3 +180,43 @@ def test_adjoint_one_side_only(gpulib):
     for t in (a, b, dc, da):
         lib.tensor_free(t)
     lib.allocator_destroy(alloc)
+
+
+def test_adjoint_repeated_calls(gpulib):
+    """Repeated calls reuse their auxiliary tensors and plans: each call
+    must follow the current data of a, b and dc, and a structure change of
+    da (a block made zero, then canonical again) or a new dc tensor, against
+    the dense gradient every time."""
+    specs = workloads.ladder_small().specs
+    idx = workloads.LADDER_IDX
+    rng = np.random.default_rng(12)
+    lib = gpulib
+    alloc = lib.allocator()
+    data = [s.random_data(rng) for s in specs[:2]]
+    dcd = specs[2].random_data(rng)
+    a = build_filled(lib, alloc, specs[0], data[0])
+    b = build_filled(lib, alloc, specs[1], data[1])
+    dc = build_filled(lib, alloc, specs[2], dcd)
+    da, db = specs[0].build(lib, alloc), specs[1].build(lib, alloc)
+    for step in range(4):
+        if step == 1:           # new values in a, b and dc, same tensors
+            data = [s.random_data(rng) for s in specs[:2]]
+            dcd = specs[2].random_data(rng)
+            specs[0].write(lib, a, data[0])
+            specs[1].write(lib, b, data[1])
+            specs[2].write(lib, dc, dcd)
+        if step == 2:           # da's structure changes (and back)
+            blk = specs[0].canonical_blocks()[0]
+            lib.set_zero(da, blk)
+            lib.set_canonical(da, blk)
+        if step == 3:           # a new dc tensor
+            lib.tensor_free(dc)
+            dcd = specs[2].random_data(rng)
+            dc = build_filled(lib, alloc, specs[2], dcd)
+        lib.contract_backward(0.75, a, b, dc, da, db, *idx)
+        want_a, want_b = dense_adjoint(specs, data, dcd, 0.75, idx)
+        assert rel_err(specs[0].read(lib, da), want_a) <= 1e-13, step
+        assert rel_err(specs[1].read(lib, db), want_b) <= 1e-13, step
+    for t in (a, b, dc, da, db):
+        lib.tensor_free(t)
+    lib.allocator_destroy(alloc)
