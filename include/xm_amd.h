@@ -88,6 +88,10 @@ typedef struct {
 	 * batch count (0: nothing staged) and the bytes copied per call */
 	int staged_batches;
 	double staged_bytes;
+	/* FP64 tiles of the GEMM launch run as two halves of their K steps
+	 * (C blocks at the end of the launch order), summed by a second
+	 * kernel */
+	size_t ksplit_tiles;
 } xm_contract_stats_t;
 
 void xm_contract_get_stats(xm_contract_stats_t *stats);

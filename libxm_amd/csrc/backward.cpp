@@ -191,6 +191,7 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 	acc->cout_bytes += st.cout_bytes;
 	acc->permute_tiled += st.permute_tiled;
 	acc->permute_gather += st.permute_gather;
+	acc->ksplit_tiles += st.ksplit_tiles;
 	acc->precision = st.precision;
 	acc->nslices = st.nslices;
 	acc->plan_cached = st.plan_cached;

@@ -496,6 +496,7 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 
 		gemm_cblock cb;
 		memset(&cb, 0, sizeof cb);
+		cb.split_pair = KSPLIT_NONE;
 		cb.p = (uint32_t)(swap ? n : m);
 		cb.q = (uint32_t)(swap ? m : n);
 		cb.ldc = std::max<uint32_t>(cb.p, 1);
@@ -695,6 +696,8 @@ plan_release(contract_plan *pl)
 			HIP_OK("xm_contract", hipEventDestroy(e));
 	if (pl->dev)
 		HIP_OK("xm_contract", hipFree(pl->dev));
+	if (pl->d_ksw)
+		HIP_OK("xm_contract", hipFree(pl->d_ksw));
 	if (pl->ws && !pl->ws_external)
 		HIP_OK("xm_contract", hipFree(pl->ws));
 	if (pl->host_scalars)
@@ -738,7 +741,8 @@ plan_upload(contract_plan *pl)
 	}
 	build_tiles(pl);
 
-	/* one device allocation: scalars | cblocks | pairs | tiles | jobs */
+	/* one device allocation: scalars | cblocks | pairs | tiles |
+	 * K-split lists | jobs */
 	std::vector<std::pair<const void *, size_t>> parts;
 	auto add = [&](const void *p, size_t n) { parts.push_back({p, n}); };
 	add(nullptr, sizeof(gemm_scalars));
@@ -747,6 +751,9 @@ plan_upload(contract_plan *pl)
 	for (int c = 0; c < pl->ncand; c++)
 		add(pl->cand_tiles[c].data(),
 		    pl->cand_tiles[c].size() * sizeof(gemm_tile));
+	for (int c = 0; c < pl->ncand; c++)
+		add(pl->cand_fix[c].data(),
+		    pl->cand_fix[c].size() * sizeof(gemm_tile));
 	const size_t esz = elem_size(pl->type);
 	std::vector<char> ju = elem_jobs_pack(pl->unfold_jobs, esz, &pl->unfold_l);
 	std::vector<char> ji = elem_jobs_pack(pl->cin_jobs, esz, &pl->cin_l);
@@ -772,9 +779,17 @@ plan_upload(contract_plan *pl)
 	pl->d_scalars = (gemm_scalars *)(base + offs[0]);
 	pl->d_cblocks = (gemm_cblock *)(base + offs[1]);
 	pl->d_pairs = (gemm_pair *)(base + offs[2]);
-	for (int c = 0; c < pl->ncand; c++)
+	size_t nslot = 0;
+	for (int c = 0; c < pl->ncand; c++) {
 		pl->d_cand_tiles[c] = (gemm_tile *)(base + offs[3 + c]);
-	const size_t oj = 3 + (size_t)pl->ncand;
+		pl->d_cand_fix[c] = (gemm_tile *)(base +
+		    offs[3 + pl->ncand + c]);
+		nslot = std::max(nslot, pl->cand_fix[c].size());
+	}
+	if (nslot)
+		HIP_OK(fn, hipMalloc(&pl->d_ksw, nslot * 2 * KSPLIT_SLOT *
+		    sizeof(double)));
+	const size_t oj = 3 + 2 * (size_t)pl->ncand;
 	pl->d_unfold = base + offs[oj];
 	pl->d_cin = base + offs[oj + 1];
 	pl->d_cout = base + offs[oj + 2];
@@ -1192,6 +1207,7 @@ xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
 		st.gemm_ms += ms[1];
 		st.post_ms += ms[2];
 		st.gemm_launches += q->gemm_launches;
+		st.ksplit_tiles += q->nfix;
 		st.panel_bytes += q->panel_bytes;
 		st.cin_bytes += q->cin_bytes;
 		st.cout_bytes += q->cout_bytes;

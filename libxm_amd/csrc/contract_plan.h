@@ -59,6 +59,10 @@ struct contract_plan {
 	int ncand = 0;
 	std::vector<gemm_tile> cand_tiles[MAX_TILE_CANDIDATES];
 	gemm_tile *d_cand_tiles[MAX_TILE_CANDIDATES] = {};
+	/* K-split tiles (first halves), one per partial-sum slot: the
+	 * sum kernel's work list */
+	std::vector<gemm_tile> cand_fix[MAX_TILE_CANDIDATES];
+	gemm_tile *d_cand_fix[MAX_TILE_CANDIDATES] = {};
 	uint32_t cand_bp[MAX_TILE_CANDIDATES] = {}, cand_bq[MAX_TILE_CANDIDATES] = {};
 	float cand_ms[MAX_TILE_CANDIDATES] = {};
 	int tile_choice = -1;		/* decided candidate, -1 while tuning */
@@ -103,6 +107,9 @@ struct contract_plan {
 	gemm_cblock *d_cblocks = nullptr;
 	gemm_pair *d_pairs = nullptr;
 	gemm_tile *d_tiles = nullptr;
+	gemm_tile *d_fix = nullptr;	/* the active candidate's */
+	size_t nfix = 0;
+	double *d_ksw = nullptr;	/* K-split partial sums */
 	void *d_unfold = nullptr, *d_cin = nullptr, *d_cout = nullptr;
 	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 

@@ -45,6 +45,103 @@ ld_raw(const double *x, uint32_t mode, uint32_t ld, uint32_t p, uint32_t k,
 }
 
 /*
+ * K split.  The tiles of a split C block run as two halves over the two
+ * parts of its (pair, 16-k step) sequence, cut at (split_pair, split_k);
+ * each half writes its sums, without beta, to its slot of partial sums
+ * (element (p, q) of the tile at p + q * BP), and k_ksplit_sum then writes
+ * C = beta * C + (half 0 + half 1).  Every kernel that runs FP64 tiles cuts
+ * a C block at the same step, so all tile shapes give identical results.
+ * With alpha == 0 both halves are empty.
+ */
+__device__ __forceinline__ void
+ksplit_range(const gemm_tile &tl, const gemm_cblock &cb, uint32_t pbeg,
+    uint32_t &sp, uint32_t &sk, uint32_t &ep, uint32_t &ek)
+{
+	sp = pbeg;
+	sk = 0;
+	ep = cb.pair_end;
+	ek = 0;
+	if (tl.ksplit && pbeg != cb.pair_end) {
+		if ((tl.ksplit - 1) & 1) {
+			sp = cb.split_pair;
+			sk = cb.split_k;
+		} else {
+			ep = cb.split_pair;
+			ek = cb.split_k;
+		}
+	}
+}
+
+/* The same as a start cursor and a count of 16-k steps. */
+__device__ __forceinline__ void
+ksplit_steps(const gemm_tile &tl, const gemm_cblock &cb, uint32_t pbeg,
+    uint32_t &sp, uint32_t &sk, uint32_t &n)
+{
+	sp = pbeg;
+	sk = 0;
+	n = pbeg == cb.pair_end ? 0 : cb.steps;
+	if (tl.ksplit && n) {
+		if ((tl.ksplit - 1) & 1) {
+			sp = cb.split_pair;
+			sk = cb.split_k;
+			n = cb.steps - cb.split_steps;
+		} else {
+			n = cb.split_steps;
+		}
+	}
+}
+
+template <int MT, int NT, int BP>
+__device__ __forceinline__ void
+ksplit_store(double *__restrict__ ksw, const gemm_tile &tl, int wp, int wq,
+    int lane, const d4 (&acc)[MT][NT])
+{
+	double *W = ksw + (size_t)(tl.ksplit - 1) * KSPLIT_SLOT;
+#pragma unroll
+	for (int i = 0; i < MT; i++)
+#pragma unroll
+		for (int j = 0; j < NT; j++)
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const int p = wp + i * 16 + (lane & 15);
+				const int q = wq + j * 16 + 4 * r + (lane >> 4);
+				W[p + q * BP] = acc[i][j][r];
+			}
+}
+
+/* C = beta * C + (half 0 + half 1) over one split tile per workgroup
+ * (fix[] holds the first halves). */
+template <int BP, int BQ>
+__global__ void __launch_bounds__(GEMM_WG)
+k_ksplit_sum(const gemm_tile *__restrict__ fix,
+    const gemm_cblock *__restrict__ cbs, const double *__restrict__ ksw,
+    const gemm_scalars *__restrict__ scp)
+{
+	const gemm_tile tl = fix[blockIdx.x];
+	const gemm_cblock cb = cbs[tl.cblock];
+	const gemm_scalars sc = *scp;
+	const double *W0 = ksw + (size_t)(tl.ksplit - 1) * KSPLIT_SLOT;
+	const double *W1 = W0 + KSPLIT_SLOT;
+	double *C = (double *)(uintptr_t)cb.c;
+	const bool bz = sc.flags & 2, none = sc.flags & 1;
+	for (int e = threadIdx.x; e < BP * BQ; e += GEMM_WG) {
+		const uint32_t p = tl.p0 + e % BP, q = tl.q0 + e / BP;
+		if (p < cb.p && q < cb.q) {
+			const size_t o = (size_t)p + (size_t)q * cb.ldc;
+			double v;
+			if (bz)
+				v = none ? 0.0 : __dadd_rn(W0[e], W1[e]);
+			else if (none)
+				v = __dmul_rn(sc.beta_re, C[o]);
+			else
+				v = __dadd_rn(__dmul_rn(sc.beta_re, C[o]),
+				    __dadd_rn(W0[e], W1[e]));
+			C[o] = v;
+		}
+	}
+}
+
+/*
  * FP64 tile kernel.  BP x BQ outputs per workgroup, 4 waves as 2 x 2, each
  * wave (BP/2) x (BQ/2) = MT x NT MFMA tiles of 16 x 16, BK k per LDS step.
  * One LDS buffer and a small register footprint keep several waves per
@@ -59,7 +156,7 @@ template <int BP, int BQ, int BK>
 __global__ void __launch_bounds__(GEMM_WG)
 k_gemm_f64(const gemm_tile *__restrict__ tiles,
     const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
-    const gemm_scalars *__restrict__ scp)
+    const gemm_scalars *__restrict__ scp, double *__restrict__ ksw)
 {
 	constexpr int MT = BP / 2 / 16, NT = BQ / 2 / 16;
 	constexpr int EX = BP * BK / GEMM_WG, EY = BQ * BK / GEMM_WG;
@@ -140,9 +237,14 @@ k_gemm_f64(const gemm_tile *__restrict__ tiles,
 
 	/* descriptors are values (scalar registers), never references */
 	const uint32_t pbeg = (sc.flags & 1) ? cb.pair_end : cb.pair_begin;
-	for (uint32_t ip = pbeg; ip < cb.pair_end; ip++) {
+	uint32_t sp, sk, ep, ek;
+	ksplit_range(tl, cb, pbeg, sp, sk, ep, ek);
+	const uint32_t elim = ek ? ep + 1 : ep;
+	for (uint32_t ip = sp; ip < elim; ip++) {
 		const gemm_pair pr = pairs[ip];
-		for (uint32_t k0 = 0; k0 < pr.k; k0 += BK) {
+		const uint32_t kb = ip == sp ? sk : 0;
+		const uint32_t ke = ip == ep ? ek : pr.k;
+		for (uint32_t k0 = kb; k0 < ke; k0 += BK) {
 			load(pr, k0);
 			__syncthreads();
 			store(pr, k0);
@@ -171,6 +273,10 @@ k_gemm_f64(const gemm_tile *__restrict__ tiles,
 
 	/* D[q][p] of v_mfma_f64_16x16x4_f64: lane -> p = lane % 16,
 	 * register r -> q = 4 * r + lane / 16 */
+	if (tl.ksplit) {
+		ksplit_store<MT, NT, BP>(ksw, tl, wp, wq, lane, acc);
+		return;
+	}
 	double *C = (double *)(uintptr_t)cb.c;
 	const bool bz = sc.flags & 2;
 	const bool none = pbeg == cb.pair_end;
@@ -236,7 +342,7 @@ template <int FD_BK, int FD_NST, int MINW>
 __global__ void __launch_bounds__(GEMM_WG, MINW)
 k_gemm_f64_dma(const gemm_tile *__restrict__ tiles,
     const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
-    const gemm_scalars *__restrict__ scp)
+    const gemm_scalars *__restrict__ scp, double *__restrict__ ksw)
 {
 	constexpr int MT = 2, NT = 4;
 	constexpr int NXD = FD_BK / 2, NYD = FD_BK;	/* X / Y DMAs per step */
@@ -303,27 +409,32 @@ k_gemm_f64_dma(const gemm_tile *__restrict__ tiles,
 		}
 	};
 
-	/* compute cursor (ip, kc, pr); fetch cursor FD_NST - 1 steps ahead */
-	uint32_t ip = pbeg, kc = 0, fip = pbeg, fk = 0;
+	/* compute cursor (ip, kc, pr) with n steps left; fetch cursor
+	 * FD_NST - 1 steps ahead with fn left */
+	uint32_t ip, kc, n;
+	ksplit_steps(tl, cb, pbeg, ip, kc, n);
+	uint32_t fip = ip, fk = kc, fn = n;
 	gemm_pair pr, fpr;
-	if (ip < pend) {
+	if (n) {
 		pr = pairs[ip];
 		fpr = pr;
 	}
 #pragma unroll
 	for (int s = 0; s < FD_NST - 1; s++)
-		if (fip < pend) {
+		if (fn) {
 			fetch(&st[s], fpr, fk);
 			advance(fip, fk, fpr);
+			fn--;
 		}
 	__builtin_amdgcn_s_waitcnt(0);
 	__syncthreads();
 	auto step = [&](const stage_t *__restrict__ cur,
 	    stage_t *__restrict__ nxt) {
-		const bool more = fip < pend;
+		const bool more = fn != 0;
 		if (more) {
 			fetch(nxt, fpr, fk);
 			advance(fip, fk, fpr);
+			fn--;
 		}
 		if (live) {
 			const double al = __dmul_rn(sc.alpha_re, pr.a_re);
@@ -357,7 +468,7 @@ k_gemm_f64_dma(const gemm_tile *__restrict__ tiles,
 		advance(ip, kc, pr);
 	};
 	int cs = 0;
-	while (ip < pend) {
+	for (; n; n--) {
 		const int ns = cs + FD_NST - 1 < FD_NST ? cs + FD_NST - 1 : cs - 1;
 		step(&st[cs], &st[ns]);
 		cs = cs + 1 < FD_NST ? cs + 1 : 0;
@@ -365,6 +476,10 @@ k_gemm_f64_dma(const gemm_tile *__restrict__ tiles,
 
 	/* D[q][p] of v_mfma_f64_16x16x4_f64: lane -> p = lane % 16,
 	 * register r -> q = 4 * r + lane / 16 */
+	if (tl.ksplit) {
+		ksplit_store<MT, NT, FD_BP>(ksw, tl, wp, wq, lane, acc);
+		return;
+	}
 	double *C = (double *)(uintptr_t)cb.c;
 	const bool bz = sc.flags & 2;
 	const bool none = pbeg == pend;
@@ -817,20 +932,18 @@ tile_signature(const contract_plan *pl)
 	return sig;
 }
 
+#define NXCD 8
+
 /*
- * Work tiles of one shape in launch order.  Workgroup w of a launch runs on
- * XCD w % 8 (round-robin dispatch) and each XCD has its own L2, so the
+ * C blocks per XCD list.  Workgroups w and w + 8 of a launch run on the
+ * same XCD (round-robin dispatch) and each XCD has its own L2, so the
  * tiles of one C block are kept on one XCD: C blocks are spread over the 8
- * XCDs by a greedy longest-first balance of their work, every XCD walks its
- * C blocks with the P-tiles innermost (tiles running together share the Y
- * panel columns and the X panel), and slot s of XCD x becomes workgroup
- * 8 * s + x.  Shorter XCD lists are padded with empty tiles.
+ * lists by a greedy longest-first balance of their work.
  */
 static void
-order_tiles(const contract_plan *pl, uint32_t BP, uint32_t BQ,
-    std::vector<gemm_tile> &tiles)
+xcd_lists(const contract_plan *pl, uint32_t BP, uint32_t BQ,
+    std::vector<uint32_t> (&lists)[NXCD])
 {
-	const int NXCD = 8;
 	std::vector<std::pair<double, uint32_t>> order;
 	for (uint32_t i = 0; i < pl->cblocks.size(); i++) {
 		const gemm_cblock &cb = pl->cblocks[i];
@@ -843,16 +956,110 @@ order_tiles(const contract_plan *pl, uint32_t BP, uint32_t BQ,
 	std::stable_sort(order.begin(), order.end(),
 	    [](const std::pair<double, uint32_t> &a,
 	    const std::pair<double, uint32_t> &b) { return a.first > b.first; });
-	std::vector<gemm_tile> bins[NXCD];
 	double load[NXCD] = {0};
 	for (auto &o : order) {
 		int x = (int)(std::min_element(load, load + NXCD) - load);
 		load[x] += o.first;
-		const gemm_cblock &cb = pl->cblocks[o.second];
-		for (uint32_t q0 = 0; q0 < cb.q; q0 += BQ)
-			for (uint32_t p0 = 0; p0 < cb.p; p0 += BP)
-				bins[x].push_back({o.second, p0, q0, 0});
+		lists[x].push_back(o.second);
 	}
+}
+
+/*
+ * K split of FP64 tiles (not with XM_GEMM_KSPLIT=0 or the FMA kernel).
+ * An XCD's list ends when its last tiles finish, and with 64 x 128 tiles
+ * of up to 18432 k (the o=64 v=384 ladder: 4470 tiles on 768 slots) the
+ * slots idled 8.6% of the GEMM's time waiting for the last ones (per-tile
+ * start and end stamps).  The C blocks holding the last KSPLIT_TAIL tiles
+ * of each list are cut in two halves of their 16-k steps, which halves the
+ * granularity where the lists drain; C blocks of fewer than
+ * KSPLIT_MIN_STEPS steps are not cut.  The lists are those of 64 x 128
+ * tiles whatever the candidates, so every tile shape cuts the same C
+ * blocks at the same step and gives the same results.
+ */
+#define KSPLIT_TAIL 96
+#define KSPLIT_MIN_STEPS 64
+
+static bool
+ksplit_enabled(const contract_plan *pl)
+{
+	const char *e = getenv("XM_GEMM_KSPLIT");
+	return !(e && *e == '0') && pl->type == XM_SCALAR_DOUBLE &&
+	    pl->precision == XM_PRECISION_NATIVE && !use_simple();
+}
+
+static void
+mark_ksplit(contract_plan *pl)
+{
+	const uint32_t BP = 64, BQ = 128;
+	for (gemm_cblock &cb : pl->cblocks) {
+		cb.split_pair = KSPLIT_NONE;
+		cb.split_k = cb.split_steps = 0;
+		uint64_t steps = 0;
+		for (uint32_t i = cb.pair_begin; i < cb.pair_end; i++)
+			steps += (pl->pairs[i].k + 15) / 16;
+		if (steps > 0xffffffffu)
+			xm_fatal("xm_contract: C block of more than 2^32 k steps");
+		cb.steps = (uint32_t)steps;
+	}
+	if (!ksplit_enabled(pl))
+		return;
+	std::vector<uint32_t> lists[NXCD];
+	xcd_lists(pl, BP, BQ, lists);
+	for (auto &l : lists) {
+		uint32_t tail = 0;
+		for (size_t n = l.size(); n-- > 0 && tail < KSPLIT_TAIL;) {
+			gemm_cblock &cb = pl->cblocks[l[n]];
+			tail += ((cb.p + BP - 1) / BP) * ((cb.q + BQ - 1) / BQ);
+			if (cb.steps < KSPLIT_MIN_STEPS)
+				continue;
+			/* first pair whose steps reach past the middle */
+			const uint32_t mid = cb.steps / 2;
+			uint32_t before = 0;
+			for (uint32_t i = cb.pair_begin; i < cb.pair_end; i++) {
+				const uint32_t ns = (pl->pairs[i].k + 15) / 16;
+				if (before + ns > mid) {
+					cb.split_pair = i;
+					cb.split_k = (mid - before) * 16;
+					cb.split_steps = mid;
+					break;
+				}
+				before += ns;
+			}
+		}
+	}
+}
+
+/*
+ * Work tiles of one shape in launch order: every XCD list walks its C
+ * blocks with the P-tiles innermost (tiles running together share the Y
+ * panel columns and the X panel), and slot s of list x becomes workgroup
+ * 8 * s + x.  Shorter lists are padded with empty tiles.  A split C
+ * block's tiles come as adjacent halves, each pair with a slot of partial
+ * sums; `fix` gets the first halves.
+ */
+static void
+order_tiles(const contract_plan *pl, uint32_t BP, uint32_t BQ,
+    std::vector<gemm_tile> &tiles, std::vector<gemm_tile> &fix)
+{
+	std::vector<uint32_t> lists[NXCD];
+	xcd_lists(pl, BP, BQ, lists);
+	std::vector<gemm_tile> bins[NXCD];
+	fix.clear();
+	for (int x = 0; x < NXCD; x++)
+		for (uint32_t i : lists[x]) {
+			const gemm_cblock &cb = pl->cblocks[i];
+			for (uint32_t q0 = 0; q0 < cb.q; q0 += BQ)
+				for (uint32_t p0 = 0; p0 < cb.p; p0 += BP) {
+					if (cb.split_pair == KSPLIT_NONE) {
+						bins[x].push_back({i, p0, q0, 0});
+						continue;
+					}
+					const uint32_t k = 1 + 2 * (uint32_t)fix.size();
+					bins[x].push_back({i, p0, q0, k});
+					bins[x].push_back({i, p0, q0, k + 1});
+					fix.push_back({i, p0, q0, k});
+				}
+		}
 	size_t per = 0;
 	for (auto &b : bins)
 		per = std::max(per, b.size());
@@ -910,10 +1117,12 @@ build_tiles(contract_plan *pl)
 	std::vector<std::pair<uint32_t, uint32_t>> cand;
 	tile_candidates(pl, cand);
 	pl->ncand = (int)std::min<size_t>(cand.size(), MAX_TILE_CANDIDATES);
+	mark_ksplit(pl);
 	for (int c = 0; c < pl->ncand; c++) {
 		pl->cand_bp[c] = cand[c].first;
 		pl->cand_bq[c] = cand[c].second;
-		order_tiles(pl, cand[c].first, cand[c].second, pl->cand_tiles[c]);
+		order_tiles(pl, cand[c].first, cand[c].second, pl->cand_tiles[c],
+		    pl->cand_fix[c]);
 		pl->cand_ms[c] = -1.0f;
 	}
 	pl->tile_sig = tile_signature(pl);
@@ -933,6 +1142,8 @@ select_tiles(contract_plan *pl)
 	pl->tile_active = c;
 	pl->tiles = pl->cand_tiles[c];
 	pl->d_tiles = pl->d_cand_tiles[c];
+	pl->d_fix = pl->d_cand_fix[c];
+	pl->nfix = pl->cand_fix[c].size();
 	pl->tile_cfg = (int)(pl->cand_bp[c] * 1000 + pl->cand_bq[c]);
 }
 
@@ -948,11 +1159,13 @@ gemm_tile_record(contract_plan *pl, float gemm_ms)
 	if (pl->tile_next > 0)
 		pl->cand_ms[pl->tile_active] = gemm_ms;
 	if (++pl->tile_next <= pl->ncand) {
-		pl->tile_active = pl->tile_next - 1;
-		pl->tiles = pl->cand_tiles[pl->tile_active];
-		pl->d_tiles = pl->d_cand_tiles[pl->tile_active];
-		pl->tile_cfg = (int)(pl->cand_bp[pl->tile_active] * 1000 +
-		    pl->cand_bq[pl->tile_active]);
+		const int c = pl->tile_next - 1;
+		pl->tile_active = c;
+		pl->tiles = pl->cand_tiles[c];
+		pl->d_tiles = pl->d_cand_tiles[c];
+		pl->d_fix = pl->d_cand_fix[c];
+		pl->nfix = pl->cand_fix[c].size();
+		pl->tile_cfg = (int)(pl->cand_bp[c] * 1000 + pl->cand_bq[c]);
 		return;
 	}
 	int best = 0;
@@ -973,20 +1186,30 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 		return;
 	dim3 g((unsigned)nt), b(GEMM_WG);
 	switch (pl->type) {
-	case XM_SCALAR_DOUBLE:
+	case XM_SCALAR_DOUBLE: {
+		const bool q128 = pl->cand_bq[pl->tile_active] == 128;
 		if (use_simple())
 			k_gemm_simple<double><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
-		else if (pl->cand_bq[pl->tile_active] == 128 && pl->f64_dma)
+		else if (q128 && pl->f64_dma)
 			k_gemm_f64_dma<16, 2, 3><<<g, b, 0, s>>>(pl->d_tiles,
-			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
-		else if (pl->cand_bq[pl->tile_active] == 128)
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars, pl->d_ksw);
+		else if (q128)
 			k_gemm_f64<64, 128, 16><<<g, b, 0, s>>>(pl->d_tiles,
-			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars, pl->d_ksw);
 		else
 			k_gemm_f64<64, 64, 16><<<g, b, 0, s>>>(pl->d_tiles,
-			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars, pl->d_ksw);
+		if (pl->nfix && q128)
+			k_ksplit_sum<64, 128><<<dim3((unsigned)pl->nfix), b, 0,
+			    s>>>(pl->d_fix, pl->d_cblocks, pl->d_ksw,
+			    pl->d_scalars);
+		else if (pl->nfix)
+			k_ksplit_sum<64, 64><<<dim3((unsigned)pl->nfix), b, 0,
+			    s>>>(pl->d_fix, pl->d_cblocks, pl->d_ksw,
+			    pl->d_scalars);
 		break;
+	}
 	case XM_SCALAR_FLOAT:
 		if (pl->mfma_dw && pl->cand_bq[pl->tile_active] == 128)
 			k_gemm_mfma_dw<float, 128, 16><<<g, b, 0, s>>>(pl->d_tiles,

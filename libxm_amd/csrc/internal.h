@@ -208,12 +208,20 @@ struct gemm_cblock {
 	uint32_t p, q, ldc;
 	uint32_t pair_begin, pair_end;
 	uint32_t flags;		/* bit0: C panel is a copy (not the block) */
+	/* K split (FP64): the tiles of a split C block run as two halves,
+	 * the first over the 16-k steps before step split_k of pair
+	 * split_pair, the second over the rest; KSPLIT_NONE if not split */
+	uint32_t split_pair, split_k;
+	uint32_t steps, split_steps;	/* 16-k steps: all, first half's */
 };
+#define KSPLIT_NONE 0xffffffffu
+/* doubles of one half's partial sums (a 64 x 128 tile at most) */
+#define KSPLIT_SLOT (64 * 128)
 
 struct gemm_tile {
 	uint32_t cblock;
 	uint32_t p0, q0;
-	uint32_t pad;
+	uint32_t ksplit;	/* 0: all of K; else 1 + 2 * slot + half */
 };
 
 /* Scalars of one contraction call, read by the kernels from device

@@ -67,7 +67,8 @@ class ContractStats(C.Structure):
                 ("tile_tuning", C.c_int), ("graph", C.c_int),
                 ("permute_tiled", C.c_size_t),
                 ("permute_gather", C.c_size_t),
-                ("staged_batches", C.c_int), ("staged_bytes", C.c_double)]
+                ("staged_batches", C.c_int), ("staged_bytes", C.c_double),
+                ("ksplit_tiles", C.c_size_t)]
 
 
 def dim(seq):

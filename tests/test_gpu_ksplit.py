@@ -1,0 +1,156 @@
+"""K split of FP64 tiles.
+
+The C blocks at the end of each XCD's tile list run as two halves of their
+16-k steps, and a second kernel writes C = beta * C + (half 0 + half 1).
+On small contractions every C block of at least 64 steps is split, so
+these tests run the split path throughout.  Results must match the oracle
+to 1e-13 relative (the two partial sums round differently from one running
+sum), keep libxm's alpha == 0 and beta == 0 rules exactly, be bit-identical
+across tile shapes and between the DMA-staged and register-staged kernels
+(every shape cuts a C block at the same step), and XM_GEMM_KSPLIT=0 turns
+the split off.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from libxm_amd import workloads
+from libxm_amd.spec import rel_err
+
+from helpers import build_filled
+
+pytestmark = pytest.mark.gpu
+
+
+def _fill(lib, w, tensors, seed):
+    for k, x in enumerate(tensors):
+        lib.fill_random(x, seed + k)
+        for ia, ib in workloads.antisym_strings(4, w.pairs):
+            lib.add(0.5, x, -0.5, x, ia, ib)
+
+
+def _workload():
+    return workloads.ladder_d2h(32, 96, 7)
+
+
+@pytest.fixture
+def env():
+    keys = ("XM_GEMM_KSPLIT", "XM_GEMM_TILE", "XM_GEMM_DMA")
+    old = {k: os.environ.get(k) for k in keys}
+    yield os.environ
+    for k, v in old.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+
+
+def test_split_matches_oracle(gpulib, oracle, env):
+    env.pop("XM_GEMM_KSPLIT", None)
+    w = _workload()
+    alloc = gpulib.allocator()
+    t, v, c = (s.build(gpulib, alloc) for s in w.specs)
+    _fill(gpulib, w, (t, v, c), 31)
+    data = [s.read(gpulib, x) for s, x in zip(w.specs, (t, v, c))]
+    oal = oracle.allocator()
+    ot = [build_filled(oracle, oal, s, d) for s, d in zip(w.specs, data)]
+    for al, be in ((0.5, 1.0), (1.5, -0.5), (0.7, 0.0), (-1.0, 2.0)):
+        gpulib.contract(al, t, v, be, c, *w.idx)
+        st = gpulib.stats()
+        assert st["ksplit_tiles"] > 0, st
+        oracle.contract(al, ot[0], ot[1], be, ot[2], *w.idx)
+        got, want = w.c.read(gpulib, c), w.c.read(oracle, ot[2])
+        assert rel_err(got, want) <= 1e-13, (al, be)
+    for x in ot:
+        oracle.tensor_free(x)
+    oracle.allocator_destroy(oal)
+    for x in (t, v, c):
+        gpulib.tensor_free(x)
+    gpulib.allocator_destroy(alloc)
+
+
+def test_split_alpha_beta_zero(gpulib, env):
+    """beta == 0 writes the products without reading C (NaN in C
+    vanishes); alpha == 0 writes beta * C exactly (signed zeros kept)."""
+    env.pop("XM_GEMM_KSPLIT", None)
+    w = _workload()
+    alloc = gpulib.allocator()
+    t, v, c = (s.build(gpulib, alloc) for s in w.specs)
+    _fill(gpulib, w, (t, v, c), 41)
+    gpulib.contract(0.5, t, v, 0.0, c, *w.idx)
+    ref = w.c.read(gpulib, c)
+    blocks = w.c.canonical_blocks()
+    for k in blocks:
+        x = np.full_like(np.asarray(ref[k]), np.nan)
+        gpulib.write_block(c, k, x)
+    gpulib.contract(0.5, t, v, 0.0, c, *w.idx)
+    assert gpulib.stats()["ksplit_tiles"] > 0
+    got = w.c.read(gpulib, c)
+    for k in blocks:
+        assert np.array_equal(np.asarray(got[k]), np.asarray(ref[k])), k
+    rng = np.random.default_rng(5)
+    before = {}
+    for k in blocks:
+        x = rng.uniform(-1, 1, np.asarray(ref[k]).shape)
+        x[x < -0.5] = -0.0
+        gpulib.write_block(c, k, x)
+        before[k] = x
+    gpulib.contract(0.0, t, v, -1.5, c, *w.idx)
+    got = w.c.read(gpulib, c)
+    for k in blocks:
+        want = -1.5 * before[k]
+        g = np.asarray(got[k])
+        assert np.array_equal(g, want), k
+        assert np.array_equal(np.signbit(g), np.signbit(want)), k
+    for x in (t, v, c):
+        gpulib.tensor_free(x)
+    gpulib.allocator_destroy(alloc)
+
+
+def test_split_identical_across_kernels(gpulib, env):
+    env.pop("XM_GEMM_KSPLIT", None)
+    w = _workload()
+    alloc = gpulib.allocator()
+    out = {}
+    for name, tile, dma in (("64x128", "64x128", None),
+                            ("64x64", "64x64", None),
+                            ("64x128 registers", "64x128", "0")):
+        env["XM_GEMM_TILE"] = tile
+        if dma is None:
+            env.pop("XM_GEMM_DMA", None)
+        else:
+            env["XM_GEMM_DMA"] = dma
+        t, v, c = (s.build(gpulib, alloc) for s in w.specs)
+        _fill(gpulib, w, (t, v, c), 51)
+        gpulib.contract(0.75, t, v, 0.5, c, *w.idx)
+        st = gpulib.stats()
+        assert st["ksplit_tiles"] > 0, (name, st)
+        assert "%dx%d" % (st["tile_p"], st["tile_q"]) == tile
+        out[name] = w.c.read(gpulib, c)
+        for x in (t, v, c):
+            gpulib.tensor_free(x)
+    gpulib.allocator_destroy(alloc)
+    first = out["64x128"]
+    for name, r in out.items():
+        for k in first:
+            assert np.array_equal(np.asarray(first[k]), np.asarray(r[k])), \
+                (name, k)
+
+
+def test_split_off(gpulib, env):
+    w = _workload()
+    alloc = gpulib.allocator()
+    res = {}
+    for flag in ("0", "1"):
+        env["XM_GEMM_KSPLIT"] = flag
+        t, v, c = (s.build(gpulib, alloc) for s in w.specs)
+        _fill(gpulib, w, (t, v, c), 61)
+        gpulib.contract(1.25, t, v, -0.5, c, *w.idx)
+        st = gpulib.stats()
+        assert (st["ksplit_tiles"] > 0) == (flag == "1"), st
+        res[flag] = w.c.read(gpulib, c)
+        for x in (t, v, c):
+            gpulib.tensor_free(x)
+    gpulib.allocator_destroy(alloc)
+    assert rel_err(res["1"], res["0"]) <= 1e-13
