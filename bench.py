@@ -14,7 +14,9 @@ GEMM work libxm itself executes).  value = F / (wall time per step) of the
 FP64 path (the drop-in); the split-precision (BF16 slices) form is reported
 beside it with the same F, its error against FP64 results, and its BF16
 matrix-core utilisation.  roofline.achieved = F / (average device time of
-the grouped GEMM kernel, HIP events on the library's stream).
+the grouped GEMM stage, HIP events on the library's stream): the GEMM
+kernel and, for the FP64 tiles run as two K halves, the kernel that sums
+the halves into C.
 
 The CPU baseline runs libxm itself (compiled from its sources into
 oracle/_ref, sequential MKL per OpenMP thread) on a bounded sample of the
@@ -422,6 +424,15 @@ def measured_traffic(kernel):
         return None
 
 
+def stage_traffic(kernels):
+    """Sum of measured_traffic over the kernels of one stage, or None if
+    the first (the stage's main kernel) has no figure."""
+    first = measured_traffic(kernels[0])
+    if first is None:
+        return None
+    return first + sum(measured_traffic(k) or 0 for k in kernels[1:])
+
+
 def permute_mode(lib, args):
     """xm_copy(b, 1, a, "ijkl", perm) on 4-D FP64 tensors: 128^4 split into
     blocks of 32 and of 16, and 112^4 split into uneven blocks of 4-28 (the
@@ -643,9 +654,12 @@ def main():
                      "peak": FP64_PEAK, "unit": "TFLOP/s",
                      "frac": F / (gemm_ms * 1e-3) / 1e12 / FP64_PEAK,
                      # the ladder's panels are all P-contiguous with even
-                     # extents: its 64 x 128 tiles run the DMA-staged kernel
-                     "traffic": measured_traffic("k_gemm_f64_dma"),
+                     # extents: its 64 x 128 tiles run the DMA-staged kernel;
+                     # the tiles split in K are summed by k_ksplit_sum
+                     "traffic": stage_traffic(("k_gemm_f64_dma",
+                                               "k_ksplit_sum")),
                      "kernel": "k_gemm_f64_dma", "kernel_ms": gemm_ms,
+                     "ksplit_tiles": st["ksplit_tiles"],
                      "tile": [st["tile_p"], st["tile_q"]],
                      "tile_tuning_in_timed_steps": bool(st["tile_tuning"])},
     }
