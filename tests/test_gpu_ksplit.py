@@ -16,9 +16,10 @@ import numpy as np
 import pytest
 
 from libxm_amd import workloads
-from libxm_amd.spec import rel_err
+from libxm_amd.spec import Spec, rel_err
 
 from helpers import build_filled
+from test_gpu_parity import run_contract
 
 pytestmark = pytest.mark.gpu
 
@@ -136,6 +137,29 @@ def test_split_identical_across_kernels(gpulib, env):
         for k in first:
             assert np.array_equal(np.asarray(first[k]), np.asarray(r[k])), \
                 (name, k)
+
+
+def test_split_with_c_panels(gpulib, oracle, env):
+    """C(i,a,j,b) = sum_k A(i,k,j) B(k,a,b): C's A and B indices
+    interleave, so C is computed in a panel and permuted back; K = 2048 in
+    four blocks makes every C block split.  The sum kernel writes the
+    panel, the fold the block."""
+    env.pop("XM_GEMM_KSPLIT", None)
+    kcut = [512, 1024, 1536]
+    specs = (Spec((16, 2048, 8), {0: [8], 1: kcut}).all_canonical(),
+             Spec((2048, 8, 16), {0: kcut}).all_canonical(),
+             Spec((16, 8, 8, 16), {0: [8]}).all_canonical())
+    idx = ("ikj", "kab", "iajb")
+    rng = np.random.default_rng(81)
+    for al, be in ((0.5, 1.0), (1.0, 0.0), (0.0, -2.0)):
+        data = [sp.random_data(rng, -1, 1) for sp in specs]
+        got, st = run_contract(gpulib, specs, data, al, be, idx, stats=True)
+        assert st["ksplit_tiles"] > 0 and st["cout_bytes"] > 0, st
+        want = run_contract(oracle, specs, data, al, be, idx)
+        assert rel_err(got, want) <= 1e-13, (al, be)
+        if al == 0.0:
+            for k in want:
+                assert np.array_equal(got[k], want[k]), k
 
 
 def test_split_off(gpulib, env):
