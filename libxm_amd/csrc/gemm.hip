@@ -983,10 +983,14 @@ xcd_lists(const contract_plan *pl, uint32_t BP, uint32_t BQ,
  * granularity where the lists drain; C blocks of fewer than
  * KSPLIT_MIN_STEPS steps are not cut.  The lists are those of 64 x 128
  * tiles whatever the candidates, so every tile shape cuts the same C
- * blocks at the same step and gives the same results.
+ * blocks at the same step and gives the same results.  Launches of more
+ * than KSPLIT_MAX_TILES such tiles (over 8 per slot of the 768 that 3
+ * workgroups on each of 256 CUs give) drain in a small share of their
+ * time and are not split.
  */
 #define KSPLIT_TAIL 96
 #define KSPLIT_MIN_STEPS 64
+#define KSPLIT_MAX_TILES (8 * 768)
 
 static bool
 ksplit_enabled(const contract_plan *pl)
@@ -1011,6 +1015,11 @@ mark_ksplit(contract_plan *pl)
 		cb.steps = (uint32_t)steps;
 	}
 	if (!ksplit_enabled(pl))
+		return;
+	size_t ntiles = 0;
+	for (const gemm_cblock &cb : pl->cblocks)
+		ntiles += (size_t)((cb.p + BP - 1) / BP) * ((cb.q + BQ - 1) / BQ);
+	if (ntiles > KSPLIT_MAX_TILES)
 		return;
 	std::vector<uint32_t> lists[NXCD];
 	xcd_lists(pl, BP, BQ, lists);
