@@ -138,6 +138,11 @@ struct seg_desc {
 	uint64_t aux;		/* per-kernel meaning */
 };
 
+/* Elements per workgroup of the fill, gather and dot kernels: a list of
+ * segments or jobs is cut into chunks of ELEM_CHUNK, first_chunk being a
+ * job's first chunk index. */
+#define ELEM_CHUNK 4096
+
 void launch_fill_value(const seg_desc *dsegs, size_t nseg, size_t nchunks,
     xm_scalar_type_t type, xm_scalar_t value, hipStream_t s);
 void launch_fill_random(const seg_desc *dsegs, size_t nseg, size_t nchunks,

@@ -14,7 +14,7 @@
  */
 #include "internal.h"
 
-#define CHUNK 4096
+#define CHUNK ELEM_CHUNK
 #define WG 256
 
 /* ---- exactly rounded scalar arithmetic ---------------------------- */

@@ -49,14 +49,15 @@ block_strides(const xm_tensor_t *t, size_t flat, uint64_t *ls)
 		ls[k] = ss[b.perm.i[k]];
 }
 
-/* Workgroups of xm_dot's per-block partial sums (4096 elements each). */
+/* Workgroups of xm_dot's per-block partial sums (ELEM_CHUNK elements
+ * each). */
 static size_t
 set_chunks(std::vector<elem_job> &jobs)
 {
 	size_t c = 0;
 	for (auto &j : jobs) {
 		j.first_chunk = c;
-		c += (j.n + 4095) / 4096;
+		c += (j.n + ELEM_CHUNK - 1) / ELEM_CHUNK;
 	}
 	return c;
 }
@@ -295,7 +296,7 @@ xm_set(xm_tensor_t *a, xm_scalar_t x)
 			seg_desc sd{a->blocks[f].data, xm_dim_dot(&d), c, f};
 			if (sd.n == 0)
 				continue;
-			c += (sd.n + 4095) / 4096;
+			c += (sd.n + ELEM_CHUNK - 1) / ELEM_CHUNK;
 			segs.push_back(sd);
 		}
 		elem_launch l;
@@ -325,7 +326,7 @@ xm_amd_fill_random(xm_tensor_t *t, uint64_t seed, double lo, double hi,
 		seg_desc sd{t->blocks[f].data, xm_dim_dot(&d), c, f};
 		if (sd.n == 0)
 			continue;
-		c += (sd.n + 4095) / 4096;
+		c += (sd.n + ELEM_CHUNK - 1) / ELEM_CHUNK;
 		segs.push_back(sd);
 	}
 	if (segs.empty())
