@@ -96,6 +96,8 @@ __device__ __forceinline__ void
 ksplit_store(double *__restrict__ ksw, const gemm_tile &tl, int wp, int wq,
     int lane, const d4 (&acc)[MT][NT])
 {
+	static_assert(BP == 32 * MT && BP * 32 * NT <= KSPLIT_SLOT,
+	    "a tile's partial sums fit one slot");
 	double *W = ksw + (size_t)(tl.ksplit - 1) * KSPLIT_SLOT;
 #pragma unroll
 	for (int i = 0; i < MT; i++)
@@ -117,6 +119,7 @@ k_ksplit_sum(const gemm_tile *__restrict__ fix,
     const gemm_cblock *__restrict__ cbs, const double *__restrict__ ksw,
     const gemm_scalars *__restrict__ scp)
 {
+	static_assert(BP * BQ <= KSPLIT_SLOT, "a tile fits one slot");
 	const gemm_tile tl = fix[blockIdx.x];
 	const gemm_cblock cb = cbs[tl.cblock];
 	const gemm_scalars sc = *scp;
