@@ -879,17 +879,8 @@ tile_candidates(const contract_plan *pl, std::vector<std::pair<uint32_t,
     uint32_t>> &out)
 {
 	if (pl->precision == XM_PRECISION_SPLIT_BF16) {
-		const char *e = getenv("XM_GEMM_TILE");
-		if (!split_staged_kernel(pl->nslices))
-			out.push_back({64, 64});
-		else if (e && strcmp(e, "128x128") == 0)
-			out.push_back({128, 128});
-		else if (e && strcmp(e, "64x128") == 0)
-			out.push_back({64, 128});
-		else {
-			out.push_back({128, 128});
-			out.push_back({64, 128});
-		}
+		out.push_back(split_staged_kernel(pl->nslices) ?
+		    std::make_pair(128u, 128u) : std::make_pair(64u, 64u));
 	} else if (pl->type == XM_SCALAR_DOUBLE && !use_simple()) {
 		const char *e = getenv("XM_GEMM_TILE");
 		if (e && strcmp(e, "64x64") == 0)

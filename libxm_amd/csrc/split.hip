@@ -237,53 +237,34 @@ frag_of(uint4 v)
 	return r;
 }
 
-/* One LDS stage: the (NPT + 4) S fragments of a k step, then the
- * exponents of the step's chunk for the tile's 32 NPT X rows and 128 Y
- * rows. */
-template <int S, int NPT>
+/* One LDS stage: the 8S fragments of a k step, then the exponents of the
+ * step's chunk for the tile's 128 X rows and 128 Y rows. */
+template <int S>
 struct sp_stage {
-	uint4 frag[(NPT + 4) * S * 64];
-	int ex[32 * NPT];
+	uint4 frag[8 * S * 64];
+	int ex[SP_BP];
 	int ey[SP_BQ];
 };
 
-/* Stages: at 128 x 128 (one workgroup per CU) as many as fit the 160 KiB
- * of LDS, up to 3; at 64 x 128 two, so that two workgroups share a CU. */
-template <int S, int NPT>
+/* stages that fit the 160 KiB of LDS */
+template <int S>
 constexpr int
 sp_nstages(void)
 {
-	return NPT == 4 && sizeof(sp_stage<S, NPT>) * 3 <= 160 * 1024 ? 3 : 2;
+	return sizeof(sp_stage<S>) * 3 <= 160 * 1024 ? 3 : 2;
 }
 
-/*
- * NPT = 4: the 128 x 128 tile above.  NPT = 2: a 64 (P) x 128 (Q) tile of
- * 4 waves, wave w owning P rows (w % 2) * 32 and Q columns (w / 2) * 64;
- * two stages, so two workgroups fit a CU and run out of step with each
- * other (one's MFMAs cover the other's barrier waits and chunk-end
- * recombinations), and P is padded to 64 rather than 128 (the o=64 v=384
- * ladder's P = 144-400 pads 22.7% of the MFMA work at 128 x 128, 8.8% at
- * 64 x 128).  The 6S fragments of a step take 1.5 S DMAs per wave (waves
- * 0-1 one more when S is odd); with two stages every wave waits for all of
- * its DMAs at the end of a step.  Per output, both forms run the same MFMA
- * and recombination sequence: results are identical.
- */
-template <int S, int NPT>
-__global__ void __launch_bounds__(64 * 2 * NPT, 2)
+template <int S>
+__global__ void __launch_bounds__(SP_WG)
 k_gemm_split2(const gemm_tile *__restrict__ tiles,
     const gemm_cblock *__restrict__ cbs, const split_pair *__restrict__ pairs,
     const gemm_scalars *__restrict__ scp)
 {
-	constexpr int NW = 2 * NPT, FX = NPT * S, NF = (NPT + 4) * S;
-	constexpr int NST = sp_nstages<S, NPT>();
-	constexpr int NCH = (NPT + 4) / 2;	/* 64-row exponent runs */
-	static_assert(NST == 2 || NF % NW == 0, "uniform DMA count per wave");
+	constexpr int FX = 4 * S, NST = sp_nstages<S>();
 	/* s_waitcnt: vmcnt = the DMAs of the NST - 2 later steps, expcnt and
 	 * lgkmcnt not waited on */
-	constexpr int WAIT = ((NST - 2) * (NF / NW + 1)) | (0x7 << 4) |
-	    (0xf << 8);
-	typedef sp_stage<S, NPT> stage_t;
-	__shared__ stage_t st[NST];
+	constexpr int WAIT = ((NST - 2) * (S + 1)) | (0x7 << 4) | (0xf << 8);
+	__shared__ sp_stage<S> st[NST];
 
 	const gemm_tile tl = tiles[blockIdx.x];
 	if (tl.cblock == 0xffffffffu)
@@ -292,7 +273,7 @@ k_gemm_split2(const gemm_tile *__restrict__ tiles,
 	const gemm_scalars sc = *scp;
 	const int tid = threadIdx.x, lane = tid & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-	const int wp = wave % NPT, wq = wave / NPT;
+	const int wp = wave & 3, wq = wave >> 2;
 	const uint32_t pw = tl.p0 + wp * 32, qw = tl.q0 + wq * 64;
 	const uint32_t pend = cb.pair_end;
 	const uint32_t pbeg = (sc.flags & 1) ? pend : cb.pair_begin;
@@ -308,48 +289,63 @@ k_gemm_split2(const gemm_tile *__restrict__ tiles,
 			acc[j][i] = 0.0;
 
 	/*
-	 * k step kx of pair px into stage nx.  Fragment f = wave + NW i is
-	 * wave's i-th (1 KiB); a row tile past the operand's padded rows
-	 * reads the zero page.  Then 64-row exponent run wave % NCH (X rows
-	 * 0-63, 64-127 up to 32 NPT, then Y rows 0-63, 64-127 of the tile) of
-	 * the step's chunk, 4 bytes a lane; waves past NCH repeat earlier
-	 * runs, writing the same values.  Rows past the padding read row 0
-	 * and are never used.
+	 * k step kx of the fetch pair into stage nx: S + 1 DMAs per wave.
+	 * Fragment f = wave + 8 i is wave's i-th (1 KiB); a row tile past the
+	 * operand's padded rows reads the zero page.  Then exponent quarter
+	 * wave % 4 (64 rows: X rows 0-63, 64-127, Y rows 0-63, 64-127 of the
+	 * tile) of the step's chunk, 4 bytes a lane; waves 4-7 repeat waves
+	 * 0-3's, writing the same values.  Rows past the padding read row 0
+	 * and are never used.  The address arithmetic is done once per pair
+	 * (setup): a step adds kx times a per-DMA stride (0 for the zero
+	 * page) to a base.  Recomputed per DMA it took ~20 scalar
+	 * instructions each, issued by both waves of a SIMD at the start of
+	 * every step while the MFMA pipe waited; hoisted, the o=64 v=384
+	 * ladder's GEMM went from 17.2 to 16.3 ms at S = 5.
 	 */
-	auto fetch = [&](stage_t *__restrict__ nx, const split_pair px,
-	    uint32_t kx) {
+	const uint4 *dsrc[S];
+	uint32_t dstp[S];
+	const int *esrc;
+	uint32_t erows;
+	const int quarter = wave & 3;
+	const bool eisx = quarter < 2;
+	auto setup = [&](const split_pair px) {
 #pragma unroll
-		for (int i = 0; i < (NF + NW - 1) / NW; i++) {
-			const int f = wave + NW * i;
-			if (NF % NW && f >= NF)
-				break;
+		for (int i = 0; i < S; i++) {
+			const int f = wave + 8 * i;
 			const bool isx = f < FX;
 			const int g = isx ? f : f - FX;
 			const uint32_t rt = (isx ? rx0 : ry0) + g / S;
 			const int t = g % S;
-			const uint4 *src = &sp_zero_page[lane];
-			if (rt * 32 < (isx ? px.xrows_pad : px.yrows_pad))
-				src = (const uint4 *)(uintptr_t)(isx ? px.xs : px.ys) +
-				    (((size_t)rt * px.nks + kx) * S + t) * 64 + lane;
+			dsrc[i] = sp_zero_page;
+			dstp[i] = 0;
+			if (rt * 32 < (isx ? px.xrows_pad : px.yrows_pad)) {
+				dsrc[i] = (const uint4 *)(uintptr_t)(isx ? px.xs :
+				    px.ys) + ((size_t)rt * px.nks * S + t) * 64;
+				dstp[i] = S * 64;
+			}
+		}
+		erows = eisx ? px.xrows_pad : px.yrows_pad;
+		uint32_t row = (eisx ? tl.p0 : tl.q0) + 64 * (quarter & 1) + lane;
+		if (row >= erows)
+			row = 0;
+		esrc = (const int *)(uintptr_t)(eisx ? px.xe : px.ye) + row;
+	};
+	auto fetch = [&](sp_stage<S> *__restrict__ nx, uint32_t kx) {
+#pragma unroll
+		for (int i = 0; i < S; i++) {
+			const int f = wave + 8 * i;
+			const uint4 *src = dsrc[i] + (size_t)kx * dstp[i] + lane;
 			__builtin_amdgcn_global_load_lds((gbl_void_t *)src,
 			    (lds_void_t *)&nx->frag[f * 64], 16, 0, 0);
 		}
-		const int ch = wave % NCH;
-		const bool isx = ch < NPT / 2;
-		const int run = isx ? ch : ch - NPT / 2;
-		const uint32_t rows = isx ? px.xrows_pad : px.yrows_pad;
-		uint32_t row = (isx ? tl.p0 : tl.q0) + 64 * run + lane;
-		if (row >= rows)
-			row = 0;
-		const int *src = (const int *)(uintptr_t)(isx ? px.xe : px.ye) +
-		    (size_t)(kx / (KC / 16)) * rows + row;
-		int *dst = (isx ? nx->ex : nx->ey) + 64 * run;
+		const int *src = esrc + (size_t)(kx / (KC / 16)) * erows;
+		int *dst = (eisx ? nx->ex : nx->ey) + 64 * (quarter & 1);
 		__builtin_amdgcn_global_load_lds((gbl_void_t *)src,
 		    (lds_void_t *)dst, 4, 0, 0);
 	};
 	/* chunk done: acc += al * 2^(ex + ey) * (((a5 + a4) + a3) + a2); a
 	 * lane's 4 consecutive q rows are one int4 of the stage's exponents */
-	auto flush = [&](const stage_t *__restrict__ cur,
+	auto flush = [&](const sp_stage<S> *__restrict__ cur,
 	    const split_pair px) {
 		const double al = __dmul_rn(sc.alpha_re, px.a_re);
 		const int ep = prow < px.xrows_pad ?
@@ -398,27 +394,38 @@ k_gemm_split2(const gemm_tile *__restrict__ tiles,
 				p = pairs[i];
 		}
 	};
+	/* the fetch cursor also sets up its new pair's DMAs */
+	auto advance_fetch = [&](uint32_t &i, uint32_t &k, split_pair &p) {
+		if (++k >= p.nks) {
+			k = 0;
+			if (++i < pend) {
+				p = pairs[i];
+				setup(p);
+			}
+		}
+	};
 	/* compute cursor (ip, ks, pr); fetch cursor NST - 1 steps ahead */
 	uint32_t ip = pbeg, ks = 0, fip = pbeg, fks = 0;
 	split_pair pr, fpr;
 	if (ip < pend) {
 		pr = pairs[ip];
 		fpr = pr;
+		setup(fpr);
 	}
 #pragma unroll
 	for (int s = 0; s < NST - 1; s++)
 		if (fip < pend) {
-			fetch(&st[s], fpr, fks);
-			advance(fip, fks, fpr);
+			fetch(&st[s], fks);
+			advance_fetch(fip, fks, fpr);
 		}
 	__builtin_amdgcn_s_waitcnt(0);
 	__syncthreads();
-	auto step = [&](const stage_t *__restrict__ cur,
-	    stage_t *__restrict__ nxt) {
+	auto step = [&](const sp_stage<S> *__restrict__ cur,
+	    sp_stage<S> *__restrict__ nxt) {
 		const bool more = fip < pend;
 		if (more) {
-			fetch(nxt, fpr, fks);
-			advance(fip, fks, fpr);
+			fetch(nxt, fks);
+			advance_fetch(fip, fks, fpr);
 		}
 		/* MFMAs of this k step: D[q][p] += Y_u * X_t by weight class */
 		bf8 fx[S];
@@ -757,18 +764,13 @@ split_gemm(contract_plan *pl, hipStream_t s)
 		SDISPATCH(k_gemm_split, ss->nslices, g, dim3(SPLIT_WG), 0, s)(t,
 		    c, ss->d_pairs, sc);
 	} else {
-		const bool p64 = pl->cand_bp[pl->tile_active] == 64;
-		const dim3 b(p64 ? SP_WG / 2 : SP_WG);
-#define SP_LAUNCH(S) (p64 ?						\
-	k_gemm_split2<S, 2><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc) :	\
-	k_gemm_split2<S, 4><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc))
+		const dim3 b(SP_WG);
 		switch (ss->nslices) {
-		case 3: SP_LAUNCH(3); break;
-		case 4: SP_LAUNCH(4); break;
-		case 5: SP_LAUNCH(5); break;
-		default: SP_LAUNCH(6); break;
+		case 3: k_gemm_split2<3><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
+		case 4: k_gemm_split2<4><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
+		case 5: k_gemm_split2<5><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
+		default: k_gemm_split2<6><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
 		}
-#undef SP_LAUNCH
 	}
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess)

@@ -100,27 +100,3 @@ def test_split_alpha_zero_and_table_case(gpulib, oracle, split):
             want = run_contract(oracle, specs, data, al, be, idx)
             assert rel_err(got, want) <= 1e-12, (fn.__name__, al, be)
 
-
-@pytest.mark.parametrize("nslices", [5, 6])
-def test_split_tile_shapes_identical(gpulib, split, nslices, monkeypatch):
-    """The 128 x 128 and 64 x 128 split tiles (pinned with XM_GEMM_TILE)
-    run the same MFMA and recombination sequence per output: identical
-    results, and each shape is the one launched."""
-    w = workloads.ladder_d2h(32, 96, 5)
-    split(nslices)
-    alloc = gpulib.allocator()
-    out = {}
-    for shape in ("128x128", "64x128"):
-        monkeypatch.setenv("XM_GEMM_TILE", shape)
-        t, v, c = (s.build(gpulib, alloc) for s in w.specs)
-        for k, x in enumerate((t, v, c)):
-            gpulib.fill_random(x, 70 + k)
-        gpulib.contract(0.5, t, v, 1.0, c, *w.idx)
-        st = gpulib.stats()
-        assert "%dx%d" % (st["tile_p"], st["tile_q"]) == shape
-        out[shape] = w.c.read(gpulib, c)
-        for x in (t, v, c):
-            gpulib.tensor_free(x)
-    gpulib.allocator_destroy(alloc)
-    for k in out["128x128"]:
-        assert np.array_equal(out["128x128"][k], out["64x128"][k]), k
