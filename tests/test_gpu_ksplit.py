@@ -162,6 +162,32 @@ def test_split_with_c_panels(gpulib, oracle, env):
                 assert np.array_equal(got[k], want[k]), k
 
 
+@pytest.mark.parametrize("dma", [None, "0"], ids=["dma", "registers"])
+def test_split_inside_pairs_of_odd_k(gpulib, oracle, env, dma):
+    """K blocks of 300, 417, 555 and 776: the middle step falls inside a
+    pair whose k is no multiple of 16, and both halves end in a partial
+    step.  The DMA-staged and the register-staged kernel, against the
+    oracle."""
+    env.pop("XM_GEMM_KSPLIT", None)
+    env.pop("XM_GEMM_TILE", None)
+    if dma is None:
+        env.pop("XM_GEMM_DMA", None)
+    else:
+        env["XM_GEMM_DMA"] = dma
+    kcut = [300, 717, 1272]
+    specs = (Spec((16, 8, 2048), {0: [8], 2: kcut}).all_canonical(),
+             Spec((2048, 8, 16), {0: kcut}).all_canonical(),
+             Spec((16, 8, 8, 16), {0: [8]}).all_canonical())
+    idx = ("ijk", "kab", "ijab")
+    rng = np.random.default_rng(91)
+    for al, be in ((0.75, -1.0), (1.0, 0.0)):
+        data = [sp.random_data(rng, -1, 1) for sp in specs]
+        got, st = run_contract(gpulib, specs, data, al, be, idx, stats=True)
+        assert st["ksplit_tiles"] > 0 and st["cout_bytes"] == 0, st
+        want = run_contract(oracle, specs, data, al, be, idx)
+        assert rel_err(got, want) <= 1e-13, (al, be)
+
+
 def test_split_off(gpulib, env):
     w = _workload()
     alloc = gpulib.allocator()
