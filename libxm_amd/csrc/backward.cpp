@@ -16,8 +16,9 @@
  * canonical blocks of dc (its derivative blocks count as zero) and G_a has
  * the nonzero blocks of a as independent canonical blocks.  Canonical
  * blocks of G_a are da's own storage; derivative blocks of a get scratch
- * storage and are then folded into their sources by the permute kernel,
- * in launch waves that never write one source block twice.
+ * storage and are then folded into their sources, each source block in
+ * one pass that reads the gradients of all its derivative blocks (up to
+ * FOLD_MAXSRC per launch) through their permutations.
  * db is computed the same way from xm_contract(alpha, a, dc', 0, G_b,
  * idxa, idxc, idxb).
  */
@@ -82,7 +83,7 @@ gradient_tensor(const xm_tensor_t *x, xm_tensor_t *dx,
  */
 struct fold_wave {
 	void *dev;			/* packed jobs, device copy */
-	elem_launch l;
+	size_t njobs, nchunks;
 };
 
 struct bwd_aux {
@@ -197,18 +198,17 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 	acc->plan_cached = st.plan_cached;
 
 	/* fold derivative blocks into their sources:
-	 * dx[S](permute(e, p)) += s * g[X](e); one launch per wave, and a
-	 * wave never writes one source block twice */
+	 * dx[S](permute(e, p)) += s * g[X](e), the X of one source in block
+	 * order; one job per source takes up to FOLD_MAXSRC of them in one
+	 * pass over dx[S] (launch g takes each source's g-th group) */
 	hipStream_t s = xm_stream();
 	if (built) {
 		for (const fold_wave &f : fold)
-			launch_elem_packed(f.dev, f.l, ELEM_ACCUM, x->type,
-			    x->type, 0, s);
+			launch_fold_packed(f.dev, f.njobs, f.nchunks, x->type, s);
 		HIP_OK(FN, hipStreamSynchronize(s));
 		return;
 	}
-	std::map<size_t, int> waves_of;	/* source -> jobs so far */
-	std::vector<std::vector<elem_job>> waves;
+	std::map<size_t, std::vector<fold_job>> groups;	/* source -> jobs */
 	for (size_t f = 0; f < x->blocks.size(); f++) {
 		const xm_block &bx = x->blocks[f];
 		if (bx.type != XM_BLOCK_TYPE_DERIVATIVE)
@@ -216,42 +216,48 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 		size_t src = (size_t)bx.data;
 		xm_dim_t sd = block_dims_flat(x, src);
 		xm_dim_t xd = block_dims_flat(x, f);
-		elem_job j;
-		memset(&j, 0, sizeof j);
-		j.dst = dx->blocks[src].data;
-		j.src = g->blocks[f].data;
-		j.n = xm_dim_dot(&sd);
-		j.ndim = (uint32_t)sd.n;
-		uint64_t xs[XM_MAX_DIM], s = 1;
+		if (xm_dim_dot(&sd) == 0)
+			continue;
+		std::vector<fold_job> &gs = groups[src];
+		if (gs.empty() || gs.back().nsrc == FOLD_MAXSRC) {
+			fold_job j;
+			memset(&j, 0, sizeof j);
+			j.dst = dx->blocks[src].data;
+			j.n = xm_dim_dot(&sd);
+			j.ndim = (uint32_t)sd.n;
+			for (size_t k = 0; k < sd.n; k++)
+				j.dims[k] = (uint32_t)sd.i[k];
+			gs.push_back(j);
+		}
+		fold_job &j = gs.back();
+		const uint32_t m = j.nsrc++;
+		j.src[m] = g->blocks[f].data;
+		uint64_t xs[XM_MAX_DIM], st = 1;
 		for (size_t k = 0; k < xd.n; k++) {
-			xs[k] = s;
-			s *= xd.i[k];
+			xs[k] = st;
+			st *= xd.i[k];
 		}
 		/* source dim perm[k] is logical dim k of X */
-		for (size_t k = 0; k < xd.n; k++) {
-			j.dims[bx.perm.i[k]] = (uint32_t)sd.i[bx.perm.i[k]];
-			j.sstride[bx.perm.i[k]] = xs[k];
-		}
-		j.f_re = bx.scalar.real();
-		j.f_im = bx.scalar.imag();
-		int w = waves_of[src]++;
-		if ((int)waves.size() <= w)
-			waves.resize(w + 1);
-		if (j.n)
-			waves[w].push_back(j);
+		for (size_t k = 0; k < xd.n; k++)
+			j.sstride[m][bx.perm.i[k]] = xs[k];
+		j.f_re[m] = bx.scalar.real();
+		j.f_im[m] = bx.scalar.imag();
 	}
+	std::vector<std::vector<fold_job>> waves;
+	for (auto &e : groups)
+		for (size_t w = 0; w < e.second.size(); w++) {
+			if (waves.size() <= w)
+				waves.resize(w + 1);
+			waves[w].push_back(e.second[w]);
+		}
 	for (auto &wv : waves) {
-		if (wv.empty())
-			continue;
-		fold_wave fw{nullptr, elem_launch()};
-		std::vector<char> pk = elem_jobs_pack(wv, elem_size(x->type),
-		    &fw.l);
+		fold_wave fw{nullptr, wv.size(), 0};
+		std::vector<char> pk = fold_jobs_pack(wv, &fw.nchunks);
 		HIP_OK(FN, hipMalloc(&fw.dev, pk.size()));
 		HIP_OK(FN, hipMemcpy(fw.dev, pk.data(), pk.size(),
 		    hipMemcpyHostToDevice));
 		fold.push_back(fw);
-		launch_elem_packed(fw.dev, fw.l, ELEM_ACCUM, x->type, x->type,
-		    0, s);
+		launch_fold_packed(fw.dev, fw.njobs, fw.nchunks, x->type, s);
 	}
 	built = true;
 	HIP_OK(FN, hipStreamSynchronize(s));

@@ -189,6 +189,24 @@ elem_launch launch_elem(const elem_job *jobs, size_t njobs, int op,
     hipStream_t s);
 bool elem_tiling_enabled(void);
 
+/* Adjoint fold of one source block: dst = ((dst + f_0 * src_0) + f_1 *
+ * src_1) + ..., each src_k a derivative-block gradient read through its
+ * permutation (strides in destination dim order), for up to FOLD_MAXSRC
+ * gradients in one pass over the destination. */
+#define FOLD_MAXSRC 4
+struct fold_job {
+	uint64_t dst, n;
+	uint32_t ndim, nsrc;
+	uint32_t dims[XM_ELEM_MAXDIM];		/* destination extents */
+	uint64_t src[FOLD_MAXSRC];
+	uint64_t sstride[FOLD_MAXSRC][XM_ELEM_MAXDIM];
+	double f_re[FOLD_MAXSRC], f_im[FOLD_MAXSRC];
+};
+std::vector<char> fold_jobs_pack(const std::vector<fold_job> &jobs,
+    size_t *nchunks);
+void launch_fold_packed(const void *djobs, size_t njobs, size_t nchunks,
+    xm_scalar_type_t type, hipStream_t s);
+
 /* Per-block dot products reduced in a fixed order (xm_dot). */
 void launch_dot(const elem_job *djobs, size_t njobs, size_t nchunks,
     xm_scalar_type_t type, double *dpartials, hipStream_t s);

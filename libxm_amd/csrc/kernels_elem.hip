@@ -823,6 +823,143 @@ launch_elem(const elem_job *jobs, size_t njobs, int op,
 	return l;
 }
 
+/* ---- adjoint folds -------------------------------------------------- */
+
+struct fold_job_dev {
+	uint64_t dst, n, first_chunk;
+	uint32_t ndim, nsrc;
+	fastdiv div[XM_ELEM_MAXDIM];	/* merged destination extents */
+	uint64_t src[FOLD_MAXSRC];
+	uint64_t sstride[FOLD_MAXSRC][XM_ELEM_MAXDIM];
+	double f_re[FOLD_MAXSRC], f_im[FOLD_MAXSRC];
+};
+
+/*
+ * Gather form over the destination (a canonical block): workgroup w of a
+ * job owns destination elements [w*CHUNK, (w+1)*CHUNK); a thread splits
+ * its element index once into the merged dims and adds every source's
+ * element in the job's order, with the arithmetic of ELEM_ACCUM, so the
+ * result equals that of one ELEM_ACCUM launch per source.  The loop over
+ * sources is unrolled to FOLD_MAXSRC (uniform predicates) so the offsets
+ * stay in registers; the loop over dims is not, so the strides are read
+ * from LDS as needed instead of being held in registers (held, they took
+ * over 200 VGPRs and left two waves per SIMD).
+ */
+template <typename T>
+__global__ void __launch_bounds__(WG)
+k_fold(const fold_job_dev *jobs, uint32_t njobs)
+{
+	const uint64_t chunk = blockIdx.x;
+	const uint32_t j = find_job(jobs, njobs, chunk);
+	__shared__ fold_job_dev jb;
+	load_job(&jb, jobs + j);
+	const uint64_t base = (chunk - jb.first_chunk) * CHUNK;
+	T *dst = (T *)(uintptr_t)jb.dst;
+	for (uint64_t i = base + threadIdx.x; i < jb.n && i < base + CHUNK;
+	    i += WG) {
+		uint64_t off[FOLD_MAXSRC] = {};
+		uint32_t q = (uint32_t)i;
+#pragma unroll 1
+		for (int d = 0; d < (int)jb.ndim; d++) {
+			const bool last = d + 1 == (int)jb.ndim;
+			const uint32_t qn = last ? 0 : fdiv_q(q, jb.div[d]);
+			const uint32_t r = last ? q : q - qn * jb.div[d].d;
+#pragma unroll
+			for (int k = 0; k < FOLD_MAXSRC; k++)
+				if (k < (int)jb.nsrc)
+					off[k] += (uint64_t)r * jb.sstride[k][d];
+			q = qn;
+		}
+		T v = dst[i];
+#pragma unroll
+		for (int k = 0; k < FOLD_MAXSRC; k++)
+			if (k < (int)jb.nsrc) {
+				const T x = ((const T *)(uintptr_t)jb.src[k])[off[k]];
+				v = eadd(v, conv<T>(emul(x,
+				    mk<T>(jb.f_re[k], jb.f_im[k]))));
+			}
+		dst[i] = v;
+	}
+}
+
+/* Device records of fold jobs (destination dims merged where every source
+ * walks them as one run) and the workgroup count of their launch. */
+std::vector<char>
+fold_jobs_pack(const std::vector<fold_job> &jobs, size_t *nchunks)
+{
+	std::vector<fold_job_dev> out;
+	out.reserve(jobs.size());
+	uint64_t c = 0;
+	for (const fold_job &a : jobs) {
+		if (a.n > 0xffffffffull)
+			XM_FATAL("libxm", "block of more than 2^32 elements");
+		if (a.nsrc == 0 || a.nsrc > FOLD_MAXSRC || a.n == 0)
+			XM_FATAL("libxm", "invalid fold job");
+		uint32_t dims[XM_ELEM_MAXDIM];
+		uint64_t str[FOLD_MAXSRC][XM_ELEM_MAXDIM];
+		uint32_t nd = 0;
+		for (uint32_t d = 0; d < a.ndim; d++) {
+			if (a.dims[d] == 1)
+				continue;	/* contributes offset 0 */
+			bool run = nd > 0 &&
+			    (uint64_t)dims[nd - 1] * a.dims[d] <= 0xffffffffu;
+			for (uint32_t k = 0; run && k < a.nsrc; k++)
+				run = str[k][nd - 1] * dims[nd - 1] ==
+				    a.sstride[k][d];
+			if (run) {
+				dims[nd - 1] *= a.dims[d];
+				continue;
+			}
+			dims[nd] = a.dims[d];
+			for (uint32_t k = 0; k < a.nsrc; k++)
+				str[k][nd] = a.sstride[k][d];
+			nd++;
+		}
+		fold_job_dev b;
+		memset(&b, 0, sizeof b);
+		b.dst = a.dst;
+		b.n = a.n;
+		b.first_chunk = c;
+		b.ndim = nd;
+		b.nsrc = a.nsrc;
+		for (uint32_t d = 0; d < XM_ELEM_MAXDIM; d++)
+			b.div[d] = make_fdiv(d < nd ? dims[d] : 1);
+		for (uint32_t k = 0; k < a.nsrc; k++) {
+			b.src[k] = a.src[k];
+			for (uint32_t d = 0; d < nd; d++)
+				b.sstride[k][d] = str[k][d];
+			b.f_re[k] = a.f_re[k];
+			b.f_im[k] = a.f_im[k];
+		}
+		c += (a.n + CHUNK - 1) / CHUNK;
+		out.push_back(b);
+	}
+	*nchunks = c;
+	std::vector<char> bytes(out.size() * sizeof(fold_job_dev));
+	if (!out.empty())
+		memcpy(bytes.data(), out.data(), bytes.size());
+	return bytes;
+}
+
+void
+launch_fold_packed(const void *djobs, size_t njobs, size_t nchunks,
+    xm_scalar_type_t type, hipStream_t s)
+{
+	if (njobs == 0 || nchunks == 0)
+		return;
+	bump_data_epoch();
+	const fold_job_dev *dj = (const fold_job_dev *)djobs;
+	const dim3 g((unsigned)nchunks), b(WG);
+	const uint32_t nj = (uint32_t)njobs;
+	switch (type) {
+	case XM_SCALAR_FLOAT: k_fold<float><<<g, b, 0, s>>>(dj, nj); break;
+	case XM_SCALAR_FLOAT_COMPLEX: k_fold<cplx<float>><<<g, b, 0, s>>>(dj, nj); break;
+	case XM_SCALAR_DOUBLE: k_fold<double><<<g, b, 0, s>>>(dj, nj); break;
+	case XM_SCALAR_DOUBLE_COMPLEX: k_fold<cplx<double>><<<g, b, 0, s>>>(dj, nj); break;
+	}
+	check_launch("xm_contract_backward fold");
+}
+
 /* XM_ELEM_TILED=0 keeps every permute in the gather form. */
 bool
 elem_tiling_enabled(void)
