@@ -667,7 +667,24 @@ mfma16(double a, double b, d4 c)
 	return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-template <typename T, int BQ, int M32_BK>
+/* One wave-wide global -> LDS DMA of DW words a lane (the size must be a
+ * literal). */
+template <int DW>
+__device__ __forceinline__ void
+lds_dma(const void *src, void *dst)
+{
+	if constexpr (DW == 4)
+		__builtin_amdgcn_global_load_lds((fd_gbl_t *)src,
+		    (fd_lds_t *)dst, 16, 0, 0);
+	else
+		__builtin_amdgcn_global_load_lds((fd_gbl_t *)src,
+		    (fd_lds_t *)dst, 4, 0, 0);
+}
+
+/* DW = 4 (FP32 plans whose panels and C blocks allow it): a lane's DMA
+ * moves 4 consecutive rows of one column, a quarter of the DMAs (and of
+ * their address arithmetic) of the dword form. */
+template <typename T, int BQ, int M32_BK, int DW = 1>
 __global__ void __launch_bounds__(GEMM_WG,
     sizeof(typename mdw<T>::R) == 8 ? 2 : 4)
 k_gemm_mfma_dw(const gemm_tile *__restrict__ tiles,
@@ -682,7 +699,9 @@ k_gemm_mfma_dw(const gemm_tile *__restrict__ tiles,
 	constexpr int NT = BQ / 32;		/* 16-column blocks per wave */
 	constexpr int XW = 64 * W, YW = BQ * W;	/* words of one column */
 	constexpr int SX = M32_BK * XW, SY = M32_BK * YW;	/* of a stage */
-	static_assert(SX % 256 == 0 && SY % 256 == 0, "DMAs per wave");
+	static_assert(SX % (256 * DW) == 0 && SY % (256 * DW) == 0,
+	    "DMAs per wave");
+	static_assert(DW == 1 || W == 1, "16-byte DMAs for FP32 only");
 	__shared__ __align__(16) float st[2][SX + SY];	/* [stage][X, Y][k][row] */
 
 	const gemm_tile tl = tiles[blockIdx.x];
@@ -707,22 +726,21 @@ k_gemm_mfma_dw(const gemm_tile *__restrict__ tiles,
 				acc[i][j][c] = acc_t{0, 0, 0, 0};
 
 	/* step at column k0 of pair px into stage nx (X words, then Y words
-	 * at SX): DMA d of an operand moves its words [64 d, 64 d + 64); wave
-	 * w issues d = w, w + 4, ... */
+	 * at SX): DMA d of an operand moves its words [64 DW d, 64 DW (d + 1));
+	 * wave w issues d = w, w + 4, ... */
 	auto fetch_op = [&](float *__restrict__ nz, const float *Z, uint32_t ld,
 	    uint32_t z0, uint32_t zn, uint32_t kn, uint32_t k0, auto cw,
 	    auto sw) {
 		constexpr int CW = decltype(cw)::value, SWD = decltype(sw)::value;
 #pragma unroll
-		for (int d = 0; d < SWD / 256; d++) {
-			const int dd = wave + 4 * d, e = dd * 64 + lane;
+		for (int d = 0; d < SWD / (256 * DW); d++) {
+			const int dd = wave + 4 * d, e = (dd * 64 + lane) * DW;
 			const uint32_t k = k0 + e / CW;
 			const uint32_t r = z0 + (e % CW) / W;
-			const void *src = (const float *)fd_zero_page + lane;
+			const void *src = (const float *)fd_zero_page + lane * DW;
 			if (k < kn && r < zn)
 				src = Z + ((size_t)r + (size_t)k * ld) * W + e % W;
-			__builtin_amdgcn_global_load_lds((fd_gbl_t *)src,
-			    (fd_lds_t *)&nz[dd * 64], 4, 0, 0);
+			lds_dma<DW>(src, &nz[dd * 64 * DW]);
 		}
 	};
 	auto fetch = [&](float *__restrict__ nx, const gemm_pair px,
@@ -1106,6 +1124,30 @@ f64_dma_eligible(const contract_plan *pl)
 	return true;
 }
 
+/* FP32 plans whose C blocks have P and Q divisible by 4 and whose panels
+ * have a leading dimension divisible by 4 and 16-byte aligned bases: what
+ * k_gemm_mfma_dw's 16-byte DMAs (4 rows of a column) need.  XM_GEMM_DMA=0
+ * keeps the dword form. */
+static bool
+mfma_x4_eligible(const contract_plan *pl)
+{
+	const char *e = getenv("XM_GEMM_DMA");
+	if (!pl->mfma_dw || pl->type != XM_SCALAR_FLOAT || (e && *e == '0'))
+		return false;
+	for (const gemm_cblock &cb : pl->cblocks) {
+		if (cb.pair_begin == cb.pair_end)
+			continue;
+		if (cb.p % 4 || cb.q % 4)
+			return false;
+		for (uint32_t i = cb.pair_begin; i < cb.pair_end; i++) {
+			const gemm_pair &g = pl->pairs[i];
+			if (g.ldx % 4 || g.ldy % 4 || g.x % 16 || g.y % 16)
+				return false;
+		}
+	}
+	return true;
+}
+
 /* Every X and Y panel P-contiguous in an FP32, complex64 or complex128
  * plan: what
  * k_gemm_mfma_dw reads.  XM_GEMM_SIMPLE=1 keeps the FMA kernel. */
@@ -1126,6 +1168,7 @@ build_tiles(contract_plan *pl)
 {
 	pl->f64_dma = f64_dma_eligible(pl);
 	pl->mfma_dw = mfma_dw_eligible(pl);
+	pl->mfma_x4 = mfma_x4_eligible(pl);
 	std::vector<std::pair<uint32_t, uint32_t>> cand;
 	tile_candidates(pl, cand);
 	pl->ncand = (int)std::min<size_t>(cand.size(), MAX_TILE_CANDIDATES);
@@ -1223,7 +1266,15 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 		break;
 	}
 	case XM_SCALAR_FLOAT:
-		if (pl->mfma_dw && pl->cand_bq[pl->tile_active] == 128)
+		if (pl->mfma_x4 && pl->cand_bq[pl->tile_active] == 128)
+			k_gemm_mfma_dw<float, 128, 16, 4><<<g, b, 0, s>>>(
+			    pl->d_tiles, pl->d_cblocks, pl->d_pairs,
+			    pl->d_scalars);
+		else if (pl->mfma_x4)
+			k_gemm_mfma_dw<float, 64, 16, 4><<<g, b, 0, s>>>(
+			    pl->d_tiles, pl->d_cblocks, pl->d_pairs,
+			    pl->d_scalars);
+		else if (pl->mfma_dw && pl->cand_bq[pl->tile_active] == 128)
 			k_gemm_mfma_dw<float, 128, 16><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		else if (pl->mfma_dw)
