@@ -18,7 +18,9 @@ CALLS = [(0.5, 1.0), (0.5, 1.0), (0.5, 1.0), (0.5, 1.0), (1.25, 0.0),
          (0.5, 1.0), (0.0, 2.0), (-0.75, 0.5), (0.5, 1.0), (1.25, 0.0)]
 
 
-def _sequence(lib, w, seed, graph):
+def _sequence(lib, w, seed, graph, splits=None):
+    """The call sequence; splits (a list) collects each call's count of
+    FP64 tiles run as two K halves."""
     lib.set_graph_launch(graph)
     alloc = lib.allocator()
     t, v, c, c2 = (s.build(lib, alloc) for s in (*w.specs, w.c))
@@ -29,7 +31,10 @@ def _sequence(lib, w, seed, graph):
     outs, replayed = [], []
     for n, (alpha, beta) in enumerate(CALLS):
         lib.contract(alpha, t, v, beta, c, *w.idx)
-        replayed.append(lib.stats()["graph"])
+        st = lib.stats()
+        replayed.append(st["graph"])
+        if splits is not None:
+            splits.append(st["ksplit_tiles"])
         outs.append(w.c.read(lib, c))
         # eager work between calls: a changed t block, then c += 0.25 c2
         idx = tblocks[rng.integers(len(tblocks))]
@@ -50,8 +55,12 @@ def _sequence(lib, w, seed, graph):
 def test_graph_replays_match_direct_launches(gpulib, make):
     w = make()
     direct, r0 = _sequence(gpulib, w, 31, False)
-    graph, r1 = _sequence(gpulib, w, 31, True)
+    splits = []
+    graph, r1 = _sequence(gpulib, w, 31, True, splits)
     assert not any(r0)
+    # the ladder's replays include the K-split sum kernel
+    if w.name.startswith("ladder"):
+        assert all(splits), splits
     # calls 0-2 tune the tile shape (direct launches); every later call of
     # the plan replays a graph
     assert all(r1[3:]), r1
