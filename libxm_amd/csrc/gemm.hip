@@ -996,13 +996,13 @@ xcd_lists(const contract_plan *pl, uint32_t BP, uint32_t BQ,
  * KSPLIT_MIN_STEPS steps are not cut.  The lists are those of 64 x 128
  * tiles whatever the candidates, so every tile shape cuts the same C
  * blocks at the same step and gives the same results.  Launches of more
- * than KSPLIT_MAX_TILES such tiles (over 8 per slot of the 768 that 3
+ * than KSPLIT_MAX_TILES such tiles (over 12 per slot of the 768 that 3
  * workgroups on each of 256 CUs give) drain in a small share of their
  * time and are not split.
  */
 #define KSPLIT_TAIL 96
 #define KSPLIT_MIN_STEPS 64
-#define KSPLIT_MAX_TILES (8 * 768)
+#define KSPLIT_MAX_TILES (12 * 768)
 
 static bool
 ksplit_enabled(const contract_plan *pl)
