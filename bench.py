@@ -19,9 +19,9 @@ kernel and, for the FP64 tiles run as two K halves, the kernel that sums
 the halves into C.
 
 The CPU baseline runs libxm itself (compiled from its sources into
-oracle/_ref, sequential MKL per OpenMP thread) on a bounded sample of the
-same workload, the canonical C blocks of the first (a,b) virtual block
-pairs, timed on the host cores as TFLOP/s of the sample's own F.
+oracle/_ref, sequential MKL per OpenMP thread) on the same workload's
+canonical C blocks (about 2 s a run on 16 host cores, median of 5 runs),
+timed on the host cores as TFLOP/s of the same F.
 """
 import argparse
 import json
@@ -79,9 +79,10 @@ def time_steps(lib, w, t, v, c, steps, warmup):
             lib.stats())
 
 
-def sample_blocks(lib, w, t, v, c):
-    """Canonical C blocks of the first (a,b) virtual block pairs, about half
-    of the workload's FLOPs, and the a/b source blocks they read."""
+def sample_blocks(lib, w, t, v, c, frac=1.0):
+    """Canonical C blocks of the first (a,b) virtual block pairs, about
+    `frac` of the workload's FLOPs (all of them by default), and the a/b
+    source blocks they read."""
     cblocks = w.c.canonical_blocks()
     abs_ = []
     for idx in cblocks:
@@ -91,11 +92,12 @@ def sample_blocks(lib, w, t, v, c):
     f_of = {}
     for p in pairs_all:
         f_of[p["c"]] = f_of.get(p["c"], 0) + 2.0 * p["m"] * p["n"] * p["k"]
-    target = 0.5 * sum(f_of.values())
+    target = frac * sum(f_of.values())
     chosen, fsum = [], 0.0
     for ab in abs_:
         blocks = [x for x in cblocks if x[2:] == ab]
-        if chosen and fsum + sum(f_of.get(x, 0) for x in blocks) > target:
+        if frac < 1.0 and chosen and \
+                fsum + sum(f_of.get(x, 0) for x in blocks) > target:
             break
         chosen += blocks
         fsum += sum(f_of.get(x, 0) for x in blocks)
@@ -109,9 +111,11 @@ def sample_blocks(lib, w, t, v, c):
     return chosen, fsum, need_a, need_b, len(cblocks)
 
 
-def cpu_baseline(lib, w, alloc_gpu, t, v, c, split_slices=None):
-    """libxm (or, if it is absent, the oracle) on a bounded sample; the same
-    sample is recomputed on the GPU (FP64 and split) for parity."""
+def cpu_baseline(lib, w, alloc_gpu, t, v, c, split_slices=None, runs=5):
+    """libxm (or, if it is absent, the oracle) on the workload's canonical C
+    blocks (about 2 s a run on 16 host cores: median of `runs`, about 10 s
+    of host work); the same blocks are recomputed on the GPU (FP64 and
+    split) for parity."""
     from libxm_amd import xm
     from libxm_amd.spec import Spec, rel_err
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libxmref.so")
@@ -134,13 +138,13 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, split_slices=None):
         cpu.write_block(cv, i, lib.read_block(v, i))
     c0 = {i: lib.read_block(c, i) for i in chosen}
     times = []
-    for _ in range(3):
+    for _ in range(runs):
         for i in chosen:
             cpu.write_block(cc, i, c0[i])
         t0 = time.perf_counter()
         cpu.contract(w.alpha, ct, cv, w.beta, cc, *w.idx)
         times.append(time.perf_counter() - t0)
-    dt = sorted(times)[1]
+    dt = sorted(times)[len(times) // 2]
     cpu_out = {i: cpu.read_block(cc, i) for i in chosen}
     errs = {}
     gc = cs.build(lib, alloc_gpu)
@@ -160,11 +164,11 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, split_slices=None):
     blas = oracle.lib.xm_oracle_blas_name().decode()
     return {"value": fsum / dt / 1e12, "unit": "TFLOP/s", "cores": ncores,
             "kind": kind,
-            "sample": "%d of %d canonical C blocks (the first %d (a,b) "
-                      "virtual block pairs), %.3g GFLOP, median of 3 runs "
-                      "%.2f s, OpenMP %d threads, BLAS %s" % (
+            "sample": "%d of %d canonical C blocks (%d (a,b) virtual "
+                      "block pairs), %.3g GFLOP, median of %d runs %.2f s, "
+                      "OpenMP %d threads, BLAS %s" % (
                           len(chosen), ncb, len({x[2:] for x in chosen}),
-                          fsum / 1e9, dt, ncores, blas),
+                          fsum / 1e9, runs, dt, ncores, blas),
             "rel_err_fp64_gpu_vs_cpu": errs[xm.PRECISION_NATIVE],
             "rel_err_split_gpu_vs_cpu": errs.get(xm.PRECISION_SPLIT_BF16)}
 
