@@ -111,11 +111,15 @@ def sample_blocks(lib, w, t, v, c, frac=1.0):
     return chosen, fsum, need_a, need_b, len(cblocks)
 
 
-def cpu_baseline(lib, w, alloc_gpu, t, v, c, split_slices=None, runs=5):
+def cpu_baseline(lib, w, alloc_gpu, t, v, c, c0, split_slices=None,
+                 runs=5):
     """libxm (or, if it is absent, the oracle) on the workload's canonical C
     blocks (about 2 s a run on 16 host cores: median of `runs`, about 10 s
     of host work); the same blocks are recomputed on the GPU (FP64 and
-    split) for parity."""
+    split) for parity.  Both start from c0, the C blocks as they were
+    before any timed call (the workload's initial c), and the errors are
+    reported relative to the whole result and relative to the contraction
+    term alone (result - beta * c0)."""
     from libxm_amd import xm
     from libxm_amd.spec import Spec, rel_err
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libxmref.so")
@@ -136,7 +140,6 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, split_slices=None, runs=5):
         cpu.write_block(ct, i, lib.read_block(t, i))
     for i in need_b:
         cpu.write_block(cv, i, lib.read_block(v, i))
-    c0 = {i: lib.read_block(c, i) for i in chosen}
     times = []
     for _ in range(runs):
         for i in chosen:
@@ -155,7 +158,8 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, split_slices=None, runs=5):
             lib.write_block(gc, i, c0[i])
         lib.contract(w.alpha, t, v, w.beta, gc, *w.idx)
         gpu_out = {i: lib.read_block(gc, i) for i in chosen}
-        errs[prec] = rel_err(gpu_out, cpu_out)
+        errs[prec] = (rel_err(gpu_out, cpu_out),
+                      rel_err_term(gpu_out, cpu_out, c0, w.beta))
     lib.set_precision(xm.PRECISION_NATIVE)
     lib.tensor_free(gc)
     for x in (ct, cv, cc):
@@ -169,8 +173,25 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, split_slices=None, runs=5):
                       "OpenMP %d threads, BLAS %s" % (
                           len(chosen), ncb, len({x[2:] for x in chosen}),
                           fsum / 1e9, runs, dt, ncores, blas),
-            "rel_err_fp64_gpu_vs_cpu": errs[xm.PRECISION_NATIVE],
-            "rel_err_split_gpu_vs_cpu": errs.get(xm.PRECISION_SPLIT_BF16)}
+            "rel_err_fp64_gpu_vs_cpu": errs[xm.PRECISION_NATIVE][0],
+            "rel_err_split_gpu_vs_cpu":
+                errs.get(xm.PRECISION_SPLIT_BF16, (None,))[0],
+            # the same differences relative to the contraction term alone
+            "rel_err_fp64_gpu_vs_cpu_product_term":
+                errs[xm.PRECISION_NATIVE][1],
+            "rel_err_split_gpu_vs_cpu_product_term":
+                errs.get(xm.PRECISION_SPLIT_BF16, (None, None))[1]}
+
+
+def rel_err_term(got, ref, c0, beta):
+    """||got - ref|| / ||ref - beta * c0|| over the blocks of ref."""
+    import numpy as np
+    num = den = 0.0
+    for i, r in ref.items():
+        r = np.asarray(r, np.complex128)
+        num += float(np.sum(np.abs(np.asarray(got[i], np.complex128) - r) ** 2))
+        den += float(np.sum(np.abs(r - beta * np.asarray(c0[i])) ** 2))
+    return (num / den) ** 0.5 if den else None
 
 
 def cpu_ring_sample(lib, w, galloc, t, v, c, c2, frac=0.1):
@@ -630,6 +651,10 @@ def main():
     log("inputs resident: %.2f GB HBM (%.1f s)" % (dev_bytes / 1e9,
                                                    time.perf_counter() - tb))
 
+    c0 = None
+    if not args.no_cpu:     # the initial C blocks of the CPU sample
+        c0 = {i: lib.read_block(c, i) for i in
+              sample_blocks(lib, w, t, v, c)[0]}
     lib.set_precision(xm.PRECISION_NATIVE)
     ms, gemm_ms, prep_ms, st = time_steps(lib, w, t, v, c, args.steps,
                                           args.warmup)
@@ -692,7 +717,7 @@ def main():
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(
-                lib, w, alloc, t, v, c,
+                lib, w, alloc, t, v, c, c0,
                 None if args.no_split else args.slices)
         except Exception as e:  # reported, never silently replaced
             result["cpu_baseline"] = {"error": repr(e)}

@@ -61,11 +61,3 @@ def ref(recorder):
 def gpulib():
     """The MI355X library.  Fails loudly if it is not built."""
     return xm.load(LIBXM_SO)
-
-
-def gpu_available():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False

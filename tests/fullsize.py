@@ -85,3 +85,46 @@ def err(gpu_blocks, ref_blocks):
 
 
 __all__ = ["fill", "pick_cblocks", "Sample", "read_blocks", "err", "xm"]
+
+
+def pick_with_largest(pairs, cblocks, budget, seed, mult=2.0):
+    """pick_cblocks, with the C block of the largest P x Q extent first."""
+    f_of, size = {}, {}
+    for p in pairs:
+        f_of[p["c"]] = f_of.get(p["c"], 0.0) + mult * p["m"] * p["n"] * p["k"]
+        size[p["c"]] = p["m"] * p["n"]
+    big = max(size, key=lambda i: (size[i], f_of[i]))
+    rest, fsum = pick_cblocks(pairs, [i for i in cblocks if i != big],
+                              budget - f_of[big], seed, mult)
+    return [big] + rest, fsum + f_of[big]
+
+
+def canonical_view(spec):
+    """Spec with spec's canonical blocks only (derivative blocks zero)."""
+    cv = Spec(spec.dims, {k: p for k, p in enumerate(spec.points)}, spec.type)
+    for idx in spec.canonical_blocks():
+        cv.canon(idx)
+    return cv
+
+
+def fold_sets(spec, sources):
+    """For each canonical block S in `sources`, the blocks X whose data is S
+    (S itself first) with X's permutation and scalar: X = s * S(perm)."""
+    out = {s: [(s, tuple(range(spec.ndim)), 1.0)] for s in sources}
+    for idx in spec.grid():
+        k = spec.kind(idx)
+        if isinstance(k, tuple) and k[1] in out:
+            out[k[1]].append((idx, k[2], k[3].real))
+    return out
+
+
+def fold_block(spec, src, members, g_blocks):
+    """Gradient w.r.t. canonical block src: sum over its members X of
+    s_X * G_X read back through X's permutation (the inverse of
+    X(e) = s * S(permute(e, p)))."""
+    acc = np.zeros(spec.block_dims(src))
+    for x, perm, s in members:
+        g = np.asarray(g_blocks[x], np.float64).reshape(spec.block_dims(x),
+                                                          order="F")
+        acc += s * np.transpose(g, np.argsort(perm))
+    return acc.ravel(order="F")

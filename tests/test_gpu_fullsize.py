@@ -7,9 +7,15 @@
   with block sizes U[4,28] (~2.5e5 block pairs in one launch, unfolds on
   both operands), in FP64 and complex64, then xm_add(1, c, 0.5, c2) and
   xm_set(c, 0).
+* Headline: the product term (beta = 0) of the o=64, v=384 ladder in FP64
+  and split precision against the oracle, on a seeded sample of C blocks
+  that includes the one with the largest P x Q extent.
 * Differentiation: xm_contract_backward of the o=64, v=384 ladder, by
   central finite differences of L = sum over canonical c of dc * c on 64
-  seeded canonical entries of t and of v, and split against FP64.
+  seeded canonical entries of t and of v, split against FP64, and against
+  the oracle's contractions of dc' (dc's canonical blocks) with v and with
+  t into every block of t / v that a sampled canonical block feeds,
+  folded on the host into the sampled blocks.
 
 The oracle recomputes a sample (fullsize.Sample) from the GPU tensors' own
 inputs, so the GPU and CPU results see the same data.
@@ -20,7 +26,9 @@ import pytest
 from libxm_amd import workloads, xm
 from libxm_amd.spec import Spec
 
-from fullsize import Sample, err, fill, pick_cblocks, read_blocks
+from fullsize import (Sample, canonical_view, err, fill, fold_block,
+                      fold_sets, pick_cblocks, pick_with_largest,
+                      read_blocks, source)
 
 pytestmark = pytest.mark.gpu
 
@@ -192,5 +200,117 @@ def test_backward_ladder_o64_v384(gpulib):
         print("backward split vs FP64:", e)
         assert e <= 1e-12
     for x in (t, v, c, dc, dcv, dt, dv, st, sv):
+        gpulib.tensor_free(x)
+    gpulib.allocator_destroy(alloc)
+
+
+def test_ladder_o64_v384_forward(gpulib, oracle):
+    w = workloads.ladder_d2h(64, 384, 0)
+    alloc = gpulib.allocator()
+    t, v, c = (s.build(gpulib, alloc) for s in w.specs)
+    for k, x in enumerate((t, v, c)):
+        fill(gpulib, w, x, k)           # the benchmark's data (seed 0)
+    pairs = gpulib.get_pairs(t, v, c, *w.idx)
+    chosen, fsum = pick_with_largest(pairs, w.c.canonical_blocks(), BUDGET,
+                                     11)
+    zero = {i: np.zeros(w.c.block_size(i)) for i in chosen}
+    ref = Sample(gpulib, oracle, w, w.specs, (t, v, c), zero, chosen,
+                 w.alpha, 0.0, w.idx, pairs)
+    errs = {}
+    for name, prec, ns in [("fp64", xm.PRECISION_NATIVE, None),
+                           ("split5", xm.PRECISION_SPLIT_BF16, 5)]:
+        gpulib.set_precision(prec, ns)
+        gpulib.contract(w.alpha, t, v, 0.0, c, *w.idx)
+        errs[name] = err(read_blocks(gpulib, c, chosen), ref.out)
+    gpulib.set_precision(xm.PRECISION_NATIVE)
+    ref.free()
+    print("ladder o=64 v=384 product term, %d of %d C blocks (%.3g GFLOP, "
+          "largest %r):" % (len(chosen), len(w.c.canonical_blocks()),
+                            fsum / 1e9, chosen[0]), errs)
+    assert errs["fp64"] <= 1e-13
+    assert errs["split5"] <= 1e-12
+    for x in (t, v, c):
+        gpulib.tensor_free(x)
+    gpulib.allocator_destroy(alloc)
+
+
+def _oracle_gradient(gpu, cpu, w, dcv, other, other_spec, g_spec, members,
+                     idx):
+    """The oracle's G = alpha * dc' (x) other into every block of `members`
+    (g_spec's block space), with dc' and the operand read from the GPU."""
+    cv = canonical_view(w.c)
+    gs = Spec(g_spec.dims, {k: p for k, p in enumerate(g_spec.points)},
+              g_spec.type)
+    for x in members:
+        gs.canon(x)
+    alloc = cpu.allocator()
+    cdc = cv.build(cpu, alloc)
+    co = other_spec.build(cpu, alloc)
+    cg = gs.build(cpu, alloc)
+    ia, ib, ic = idx
+    need_dc, need_o = set(), set()
+    # g = "ijcd" (dt) or "abcd" (dv); dc' = "ijab"; other = "abcd" / "ijcd"
+    for x in members:
+        for d in cv.canonical_blocks():
+            if (ic == "ijcd" and d[:2] == x[:2]) or \
+                    (ic == "abcd" and d[2:] == x[:2]):
+                need_dc.add(d)
+                oi = d[2:] + x[2:] if ic == "ijcd" else d[:2] + x[2:]
+                if other_spec.kind(oi) is not None:
+                    need_o.add(source(other_spec, oi))
+    for d in need_dc:
+        cpu.write_block(cdc, d, gpu.read_block(dcv, d))
+    for d in need_o:
+        cpu.write_block(co, d, gpu.read_block(other, d))
+    cpu.contract(w.alpha, cdc, co, 0.0, cg, ia, ib, ic)
+    out = {x: cpu.read_block(cg, x) for x in members}
+    for x in (cdc, co, cg):
+        cpu.tensor_free(x)
+    cpu.allocator_destroy(alloc)
+    return out
+
+
+def test_backward_ladder_o64_v384_vs_oracle(gpulib, oracle):
+    w = workloads.ladder_d2h(64, 384, 0)
+    alloc = gpulib.allocator()
+    t, v, c, dc = (s.build(gpulib, alloc) for s in (*w.specs, w.c))
+    for k, x in enumerate((t, v, c, dc)):
+        fill(gpulib, w, x, 400 + k)
+    dcv = canonical_view(w.c).build(gpulib, alloc)
+    gpulib.copy(dcv, 1.0, dc, "ijab", "ijab")
+    rng = np.random.default_rng(17)
+
+    def sample(spec, n_off, n_diag):
+        cb = spec.canonical_blocks()
+        off = [i for i in cb if i[0] < i[1] and i[2] < i[3]]
+        diag = [i for i in cb if i[0] == i[1] or i[2] == i[3]]
+        return [off[k] for k in rng.choice(len(off), n_off, replace=False)] \
+            + [diag[k] for k in rng.choice(len(diag), n_diag, replace=False)]
+
+    # dt: 15 GFLOP per t block on the host; dv: ~3 GFLOP per v block
+    plan = ((w.t, t, v, w.v, sample(w.t, 2, 1), ("ijab", "abcd", "ijcd")),
+            (w.v, v, t, w.t, sample(w.v, 6, 2), ("ijab", "ijcd", "abcd")))
+    refs = []
+    for spec, x, other, ospec, srcs, idx in plan:
+        fs = fold_sets(spec, srcs)
+        members = [m[0] for s in srcs for m in fs[s]]
+        g = _oracle_gradient(gpulib, oracle, w, dcv, other, ospec, spec,
+                             members, idx)
+        refs.append({s: fold_block(spec, s, fs[s], g) for s in srcs})
+    for name, prec, ns, tol in (("fp64", xm.PRECISION_NATIVE, None, 1e-13),
+                                ("split5", xm.PRECISION_SPLIT_BF16, 5, 1e-12)):
+        gpulib.set_precision(prec, ns)
+        dt, dv = w.t.build(gpulib, alloc), w.v.build(gpulib, alloc)
+        gpulib.contract_backward(w.alpha, t, v, dc, dt, dv, *w.idx)
+        for (spec, *_), g, ref in zip(plan, (dt, dv), refs):
+            e = err(read_blocks(gpulib, g, list(ref)), ref)
+            print("backward %s vs oracle contractions, %d canonical %s "
+                  "blocks: rel %.2e" % (name, len(ref),
+                                        "t" if spec is w.t else "v", e))
+            assert e <= tol
+        gpulib.tensor_free(dt)
+        gpulib.tensor_free(dv)
+    gpulib.set_precision(xm.PRECISION_NATIVE)
+    for x in (t, v, c, dc, dcv):
         gpulib.tensor_free(x)
     gpulib.allocator_destroy(alloc)

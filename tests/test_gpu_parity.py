@@ -145,14 +145,11 @@ def test_set_copy_mul_div_dot(t, gpulib, oracle):
     alloc = gpulib.allocator()
     a = sp.build(gpulib, alloc)
     gpulib.set(a, x)
+    want = x if np.dtype(xm.NP_DTYPE[t]).kind == "c" else x.real
     for v in sp.read(gpulib, a).values():
-        assert np.all(v == np.asarray(x).astype(v.dtype))
+        assert np.all(v == np.asarray(want).astype(v.dtype))
     gpulib.tensor_free(a)
     gpulib.allocator_destroy(alloc)
-    # copy with transposition, scale, derivative source blocks and into
-    # the other precision
-    for fn, ia, ib in [(lambda tt: _ab3(tt), "ji", "ij")]:
-        pass
     for fn, ia, ib in MULDIV_TESTS:
         sa, sb = fn(t)
         data = make_data((sa, sb), 31)
@@ -194,9 +191,25 @@ def test_set_copy_mul_div_dot(t, gpulib, oracle):
         assert abs(res[0] - res[1]) <= (1e-4 if is_single(t) else 1e-12) * scale
 
 
-def _ab3(t):
+@pytest.mark.parametrize("t", TYPES, ids=[TYPE_NAMES[t] for t in TYPES])
+def test_copy_transposed_scaled(t, gpulib, oracle):
+    """xm_copy(a, 0.75, b, "ji", "ij") from a b holding a derivative block
+    (scalar -4, transposed source; libxm test.c:458-477), bit-exact."""
     from cases import ab_3
-    return ab_3(t)
+    sa, sb = ab_3(t)
+    data = make_data((sa, sb), 43)
+    outs = []
+    for lib in (gpulib, oracle):
+        al = lib.allocator()
+        aa = build_filled(lib, al, sa, data[0])
+        bb = build_filled(lib, al, sb, data[1])
+        lib.copy(aa, 0.75, bb, "ji", "ij")
+        outs.append(sa.read(lib, aa))
+        lib.tensor_free(aa)
+        lib.tensor_free(bb)
+        lib.allocator_destroy(al)
+    for k in outs[1]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
 
 
 @pytest.mark.parametrize("t", [xm.SCALAR_FLOAT, xm.SCALAR_DOUBLE],
