@@ -137,6 +137,24 @@ void xm_contract_backward(xm_scalar_t alpha, const xm_tensor_t *a,
     xm_tensor_t *db, const char *idxa, const char *idxb, const char *idxc);
 
 /*
+ * xm_contract without any device allocation.  No libxm counterpart: it is
+ * for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, checks and result as
+ * xm_contract in native precision (whatever xm_set_contract_precision
+ * says), with one tile shape and no K split, so results match
+ * xm_contract's to FP64 rounding (FP64: within 1e-13 relative).  Every
+ * device the library runs on holds a 64 MiB array for the call's permuted
+ * operand and C panels and its descriptors until the process ends; calls
+ * on one device share it, so calls that overlap on one device (from
+ * several host threads) race.  Returns 0, or -1 after a message on stderr,
+ * computing nothing, when the contraction needs more than the array or
+ * touches blocks in pinned host memory (those need staging buffers).
+ */
+int xm_contract_noalloc(xm_scalar_t alpha, const xm_tensor_t *a,
+    const xm_tensor_t *b, xm_scalar_t beta, xm_tensor_t *c, const char *idxa,
+    const char *idxb, const char *idxc);
+
+/*
  * Fill every canonical block of t with reproducible pseudo-random values
  * generated on the device: x = u * 10^w, u ~ U[lo, hi), w ~ U[wlo, whi)
  * (wlo == whi == 0 gives plain uniform values).  Real and imaginary parts

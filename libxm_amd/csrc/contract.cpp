@@ -712,15 +712,11 @@ plan_release(contract_plan *pl)
 	delete pl;
 }
 
-/* Relocate workspace offsets to addresses and move every descriptor list
- * to one device allocation. */
+/* Turn the plan's workspace offsets into addresses: the workspace starts
+ * at wsb. */
 static void
-plan_upload(contract_plan *pl)
+plan_relocate(contract_plan *pl, uint64_t wsb)
 {
-	const char *fn = "xm_contract";
-	if (pl->ws_bytes && !pl->ws_external)
-		HIP_OK(fn, hipMalloc(&pl->ws, pl->ws_bytes));
-	uint64_t wsb = (uint64_t)(uintptr_t)pl->ws;
 	for (size_t i = 0; i < pl->unfold_jobs.size(); i++) {
 		pl->unfold_jobs[i].dst += wsb;
 		if (pl->unfold_src_ws[i])
@@ -739,6 +735,17 @@ plan_upload(contract_plan *pl)
 		if (pl->y_in_ws[i])
 			pl->pairs[i].y += wsb;
 	}
+}
+
+/* Relocate workspace offsets to addresses and move every descriptor list
+ * to one device allocation. */
+static void
+plan_upload(contract_plan *pl)
+{
+	const char *fn = "xm_contract";
+	if (pl->ws_bytes && !pl->ws_external)
+		HIP_OK(fn, hipMalloc(&pl->ws, pl->ws_bytes));
+	plan_relocate(pl, (uint64_t)(uintptr_t)pl->ws);
 	build_tiles(pl);
 
 	/* one device allocation: scalars | cblocks | pairs | tiles |
@@ -991,11 +998,14 @@ plan_cache_clear(void)
 	cache.clear();
 }
 
+static void noalloc_cache_drop(uint64_t id);
+
 /* Drop the cached plans that reference tensor `id` (the tensor is being
  * freed: its plans can never be reused and hold device memory). */
 void
 plan_cache_drop(uint64_t id)
 {
+	noalloc_cache_drop(id);
 	std::lock_guard<std::mutex> g(cache_mu);
 	for (auto it = cache.begin(); it != cache.end();) {
 		const plan_key &k = it->first;
@@ -1272,4 +1282,222 @@ xm_contract_get_pairs(const xm_tensor_t *a, const xm_tensor_t *b,
 	last_stats.nunfold = pl->nunfold;
 	delete pl;
 	return n;
+}
+
+/* ---- contraction without device allocation ------------------------ */
+
+/*
+ * xm_contract_noalloc keeps what a native-precision plan needs on the
+ * device beyond the tensors' own blocks (permuted operand and C panels,
+ * then the scalars, C-block, pair and tile descriptors and the packed
+ * permute jobs) in one file-scope array per device, `scratch`.  In host
+ * code the array's name is not a device address: its address on the
+ * current device comes from hipGetSymbolAddress, once per device, and
+ * only that address is used (kernel arguments, copies).  Every call on a
+ * device shares that device's array and leaves its descriptors there;
+ * nothing in it needs clearing: each call copies its descriptors in, and
+ * the permutes write every panel element before the GEMM reads it.  Calls
+ * that overlap on one device (from several host threads) race on it.
+ */
+#define NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+#define NOALLOC_MAX_DEVICES 64
+
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+static std::mutex noalloc_mu;
+static void *noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+/* Address of `scratch` on the current device (looked up on the first call
+ * with that device current). */
+static char *
+noalloc_scratch(const char *fn, int *dev)
+{
+	HIP_OK(fn, hipGetDevice(dev));
+	if (*dev < 0 || *dev >= NOALLOC_MAX_DEVICES)
+		XM_FATAL(fn, "device index out of range");
+	std::lock_guard<std::mutex> g(noalloc_mu);
+	if (noalloc_ptr[*dev] == nullptr) {
+		void *p = nullptr;
+		HIP_OK(fn, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		if (p == nullptr)
+			XM_FATAL(fn, "no device address for the scratch array");
+		noalloc_ptr[*dev] = p;
+	}
+	return (char *)noalloc_ptr[*dev];
+}
+
+/* A host-only plan laid out for one device's scratch array: panels at
+ * offset 0, then the descriptor image (copied in by every call). */
+struct noalloc_plan {
+	contract_plan *pl = nullptr;
+	std::vector<char> image;
+	size_t doff = 0;	/* scratch offset of the image */
+	size_t need = 0;	/* scratch bytes the call needs */
+};
+
+struct noalloc_key {
+	plan_key k;
+	int dev;
+	bool operator<(const noalloc_key &o) const {
+		if (dev != o.dev)
+			return dev < o.dev;
+		return k < o.k;
+	}
+};
+
+static std::list<std::pair<noalloc_key, noalloc_plan>> noalloc_cache;
+
+static void
+noalloc_cache_drop(uint64_t id)
+{
+	std::lock_guard<std::mutex> g(noalloc_mu);
+	for (auto it = noalloc_cache.begin(); it != noalloc_cache.end();) {
+		const plan_key &k = it->first.k;
+		if (k.ida == id || k.idb == id || k.idc == id) {
+			delete it->second.pl;
+			it = noalloc_cache.erase(it);
+		} else {
+			++it;
+		}
+	}
+}
+
+/* The plan of c = alpha a b + beta c with its device state in `base`;
+ * np.need says how much of the array it takes (the caller checks it). */
+static void
+noalloc_build(const xm_tensor_t *a, const xm_tensor_t *b,
+    const xm_tensor_t *c, const masks_t &mk, char *base, noalloc_plan &np)
+{
+	contract_plan *pl = build_plan(a, b, c, mk, XM_PRECISION_NATIVE);
+	pl->single_tile = pl->no_ksplit = true;
+	const uint64_t wsb = (uint64_t)(uintptr_t)base;
+	plan_relocate(pl, wsb);
+	build_tiles(pl);
+	const size_t esz = elem_size(pl->type);
+	std::vector<char> ju = elem_jobs_pack(pl->unfold_jobs, esz, &pl->unfold_l);
+	std::vector<char> ji = elem_jobs_pack(pl->cin_jobs, esz, &pl->cin_l);
+	std::vector<char> jo = elem_jobs_pack(pl->cout_jobs, esz, &pl->cout_l);
+	std::vector<std::pair<const void *, size_t>> parts = {
+		{nullptr, sizeof(gemm_scalars)},
+		{pl->cblocks.data(), pl->cblocks.size() * sizeof(gemm_cblock)},
+		{pl->pairs.data(), pl->pairs.size() * sizeof(gemm_pair)},
+		{pl->cand_tiles[0].data(),
+		    pl->cand_tiles[0].size() * sizeof(gemm_tile)},
+		{ju.data(), ju.size()}, {ji.data(), ji.size()},
+		{jo.data(), jo.size()}};
+	np.doff = (pl->ws_bytes + 255) / 256 * 256;
+	std::vector<size_t> offs;
+	size_t total = 0;
+	for (auto &p : parts) {
+		offs.push_back(total);
+		total += (p.second + 255) / 256 * 256;
+	}
+	np.need = np.doff + total;
+	np.pl = pl;
+	if (np.need > NOALLOC_SCRATCH_BYTES)
+		return;
+	np.image.assign(total, 0);
+	for (size_t i = 0; i < parts.size(); i++)
+		if (parts[i].first && parts[i].second)
+			memcpy(np.image.data() + offs[i], parts[i].first,
+			    parts[i].second);
+	char *d = base + np.doff;
+	pl->d_scalars = (gemm_scalars *)(d + offs[0]);
+	pl->d_cblocks = (gemm_cblock *)(d + offs[1]);
+	pl->d_pairs = (gemm_pair *)(d + offs[2]);
+	pl->d_cand_tiles[0] = (gemm_tile *)(d + offs[3]);
+	pl->d_cand_fix[0] = nullptr;
+	pl->d_unfold = d + offs[4];
+	pl->d_cin = d + offs[5];
+	pl->d_cout = d + offs[6];
+	select_tiles(pl);
+}
+
+XM_API int
+xm_contract_noalloc(xm_scalar_t alpha, const xm_tensor_t *a,
+    const xm_tensor_t *b, xm_scalar_t beta, xm_tensor_t *c, const char *idxa,
+    const char *idxb, const char *idxc)
+{
+	const char *fn = "xm_contract_noalloc";
+	auto t0 = std::chrono::steady_clock::now();
+	masks_t mk;
+	contract_masks(fn, a, b, c, idxa, idxb, idxc, mk);
+	if (touches_host(a, b, c)) {
+		fprintf(stderr, "libxm: %s: blocks in host memory need staging "
+		    "buffers; use xm_contract\n", fn);
+		return -1;
+	}
+	int dev = 0;
+	char *base = noalloc_scratch(fn, &dev);
+	noalloc_key key{plan_key{a->id, a->version, b->id, b->version, c->id,
+	    c->version, idxa, idxb, idxc, XM_PRECISION_NATIVE, 0}, dev};
+	std::unique_lock<std::mutex> g(noalloc_mu);
+	auto it = noalloc_cache.begin();
+	for (; it != noalloc_cache.end(); ++it)
+		if (!(it->first < key) && !(key < it->first))
+			break;
+	bool cached = it != noalloc_cache.end();
+	if (cached) {
+		noalloc_cache.splice(noalloc_cache.begin(), noalloc_cache, it);
+	} else {
+		noalloc_plan np;
+		noalloc_build(a, b, c, mk, base, np);
+		noalloc_cache.push_front({key, np});
+		while (noalloc_cache.size() > CACHE_MAX) {
+			delete noalloc_cache.back().second.pl;
+			noalloc_cache.pop_back();
+		}
+	}
+	noalloc_plan &np = noalloc_cache.front().second;
+	if (np.need > NOALLOC_SCRATCH_BYTES) {
+		fprintf(stderr, "libxm: %s: the contraction needs %zu bytes of "
+		    "device scratch (panels and descriptors), more than the %zu "
+		    "of the no-allocation path; use xm_contract\n", fn, np.need,
+		    NOALLOC_SCRATCH_BYTES);
+		return -1;
+	}
+	contract_plan *pl = np.pl;
+	hipStream_t s = xm_stream();
+	bump_data_epoch();
+	gemm_scalars sc;
+	memset(&sc, 0, sizeof sc);
+	sc.alpha_re = alpha.real();
+	sc.alpha_im = alpha.imag();
+	sc.beta_re = beta.real();
+	sc.beta_im = beta.imag();
+	sc.flags = alpha == xm_scalar_t(0) ? 1 : 0;
+	sc.flags |= beta == xm_scalar_t(0) ? 2 : 0;
+	memcpy(np.image.data(), &sc, sizeof sc);
+	HIP_OK(fn, hipMemcpyAsync(base + np.doff, np.image.data(),
+	    np.image.size(), hipMemcpyHostToDevice, s));
+	if (!(sc.flags & 1))
+		launch_elem_packed(pl->d_unfold, pl->unfold_l, ELEM_GATHER,
+		    pl->type, pl->type, 0, s);
+	if (!(sc.flags & 2))
+		launch_elem_packed(pl->d_cin, pl->cin_l, ELEM_GATHER, pl->type,
+		    pl->type, 0, s);
+	launch_gemm(pl, s);
+	launch_elem_packed(pl->d_cout, pl->cout_l, ELEM_GATHER, pl->type,
+	    pl->type, 0, s);
+	HIP_OK(fn, hipStreamSynchronize(s));
+	auto t1 = std::chrono::steady_clock::now();
+	xm_contract_stats_t &st = last_stats;
+	memset(&st, 0, sizeof st);
+	st.flops = pl->flops;
+	st.flops_pre = pl->flops_pre;
+	st.npairs = pl->npairs;
+	st.npairs_pre = pl->npairs_pre;
+	st.nblocks_c = pl->nblocks_c;
+	st.nunfold = pl->nunfold;
+	st.unfold_bytes = pl->unfold_bytes;
+	st.panel_bytes = pl->panel_bytes;
+	st.cin_bytes = pl->cin_bytes;
+	st.cout_bytes = pl->cout_bytes;
+	st.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+	st.plan_cached = cached;
+	st.precision = XM_PRECISION_NATIVE;
+	st.gemm_launches = pl->gemm_launches;
+	st.tile_p = (int)pl->cand_bp[0];
+	st.tile_q = (int)pl->cand_bq[0];
+	return 0;
 }

@@ -72,6 +72,10 @@ struct contract_plan {
 	bool mfma_dw = false;		/* k_gemm_mfma_dw can run the plan */
 	bool mfma_x4 = false;		/* ... with 16-byte DMAs (FP32) */
 	bool even_extents = false;	/* every C block's P and Q even */
+	/* plans of xm_contract_noalloc: one tile shape (the first
+	 * candidate, never tuned) and no K split, whose partial sums would
+	 * need device memory of their own */
+	bool single_tile = false, no_ksplit = false;
 	std::string tile_sig;		/* block-size histogram signature */
 
 	/* permute jobs: operand panels, C in, C out */

@@ -923,6 +923,8 @@ tile_candidates(const contract_plan *pl, std::vector<std::pair<uint32_t,
 	} else {
 		out.push_back({64, 64});
 	}
+	if (pl->single_tile)
+		out.resize(1);
 }
 
 /* decided candidate per block-size histogram signature */
@@ -1008,7 +1010,8 @@ static bool
 ksplit_enabled(const contract_plan *pl)
 {
 	const char *e = getenv("XM_GEMM_KSPLIT");
-	return !(e && *e == '0') && pl->type == XM_SCALAR_DOUBLE &&
+	return !(e && *e == '0') && !pl->no_ksplit &&
+	    pl->type == XM_SCALAR_DOUBLE &&
 	    pl->precision == XM_PRECISION_NATIVE && !use_simple();
 }
 

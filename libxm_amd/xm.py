@@ -188,6 +188,7 @@ _SIGNATURES = {
     "xm_set_graph_launch": (None, [C.c_int]),
     "xm_get_graph_launch": (C.c_int, []),
     "xm_contract_backward": (None, [Scalar, _P, _P, _P, _P, _P, _S, _S, _S]),
+    "xm_contract_noalloc": (C.c_int, [Scalar, _P, _P, Scalar, _P, _S, _S, _S]),
     "xm_amd_fill_random": (None, [_P, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double]),
     "xm_amd_allocator_usage": (None, [_P, C.POINTER(_SZ), C.POINTER(_SZ)]),
     "xm_amd_synchronize": (C.c_int, []),
@@ -203,7 +204,8 @@ CORE_SYMBOLS = [n for n in _SIGNATURES
                               "xm_get_contract_precision",
                               "xm_set_split_slices", "xm_get_split_slices",
                               "xm_set_graph_launch", "xm_get_graph_launch",
-                              "xm_contract_backward")]
+                              "xm_contract_backward",
+                              "xm_contract_noalloc")]
 
 _libc = C.CDLL(None)
 _libc.free.argtypes = [C.c_void_p]
@@ -364,6 +366,10 @@ class XmLib:
 
     def dot(self, a, b, ia, ib):
         return unscalar(self.lib.xm_dot(a, b, _b(ia), _b(ib)))
+
+    def contract_noalloc(self, alpha, a, b, beta, c, ia, ib, ic):
+        return self.lib.xm_contract_noalloc(scalar(alpha), a, b, scalar(beta),
+                                            c, _b(ia), _b(ib), _b(ic))
 
     def contract_backward(self, alpha, a, b, dc, da, db, ia, ib, ic):
         self.lib.xm_contract_backward(scalar(alpha), a, b, dc, da, db,
