@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK = 78.6          # TFLOP/s, MI355X dense FP64 matrix
 BF16_PEAK = 2516.6        # TFLOP/s, MI355X dense BF16 matrix
+INT8_PEAK = 5033.2        # TOP/s, MI355X dense int8 matrix
 HBM_PEAK = 8000.0         # GB/s
 
 
@@ -152,7 +153,8 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, c0, split_slices=None,
     errs = {}
     gc = cs.build(lib, alloc_gpu)
     for prec, ns in [(xm.PRECISION_NATIVE, None)] + (
-            [(xm.PRECISION_SPLIT_BF16, split_slices)] if split_slices else []):
+            [(xm.PRECISION_SPLIT_BF16, split_slices),
+             (xm.PRECISION_SPLIT_INT8, None)] if split_slices else []):
         lib.set_precision(prec, ns)
         for i in chosen:
             lib.write_block(gc, i, c0[i])
@@ -180,7 +182,11 @@ def cpu_baseline(lib, w, alloc_gpu, t, v, c, c0, split_slices=None,
             "rel_err_fp64_gpu_vs_cpu_product_term":
                 errs[xm.PRECISION_NATIVE][1],
             "rel_err_split_gpu_vs_cpu_product_term":
-                errs.get(xm.PRECISION_SPLIT_BF16, (None, None))[1]}
+                errs.get(xm.PRECISION_SPLIT_BF16, (None, None))[1],
+            "rel_err_int8_gpu_vs_cpu":
+                errs.get(xm.PRECISION_SPLIT_INT8, (None,))[0],
+            "rel_err_int8_gpu_vs_cpu_product_term":
+                errs.get(xm.PRECISION_SPLIT_INT8, (None, None))[1]}
 
 
 def rel_err_term(got, ref, c0, beta):
@@ -275,14 +281,16 @@ def cpu_ring_sample(lib, w, galloc, t, v, c, c2, frac=0.1):
     return out
 
 
-def split_accuracy(lib, w, alloc, t, v, nslices):
+def split_accuracy(lib, w, alloc, t, v, nslices,
+                   precision=None):
     """Product term alone (beta = 0): split vs FP64 on the whole workload."""
     from libxm_amd import xm
     from libxm_amd.spec import rel_err
     c1, c2 = w.c.build(lib, alloc), w.c.build(lib, alloc)
     lib.set_precision(xm.PRECISION_NATIVE)
     lib.contract(w.alpha, t, v, 0.0, c1, *w.idx)
-    lib.set_precision(xm.PRECISION_SPLIT_BF16, nslices)
+    lib.set_precision(xm.PRECISION_SPLIT_BF16 if precision is None
+                      else precision, nslices)
     lib.contract(w.alpha, t, v, 0.0, c2, *w.idx)
     lib.set_precision(xm.PRECISION_NATIVE)
     err = rel_err(w.c.read(lib, c2), w.c.read(lib, c1))
@@ -714,6 +722,32 @@ def main():
                          "frac": F * nprod / (sgemm * 1e-3) / 1e12 / BF16_PEAK,
                          "traffic": measured_traffic("k_gemm_split2"),
                          "kernel": "k_gemm_split2", "kernel_ms": sgemm}}
+    if not args.no_split:
+        lib.set_precision(xm.PRECISION_SPLIT_INT8, None, 0)
+        ims, igemm, iprep, ist = time_steps(lib, w, t, v, c, args.steps,
+                                            args.warmup)
+        lib.set_precision(xm.PRECISION_NATIVE)
+        nmod = ist["nslices"]
+        ierr = split_accuracy(lib, w, alloc, t, v, None,
+                              xm.PRECISION_SPLIT_INT8)
+        log("int8 CRT: %.3f ms/step (residues %.3f, gemm %.3f), %.2f "
+            "TFLOP/s, %d moduli, rel err %.2e" % (
+                ims, iprep, igemm, F / ims / 1e9, nmod, ierr))
+        result["split_int8"] = {
+            "value": F / (ims * 1e-3) / 1e12,
+            "unit": "TFLOP/s (FP64-equivalent, same F)",
+            "ms_per_step": ims, "residue_ms": iprep, "gemm_ms": igemm,
+            "moduli": nmod, "int8_products_per_fp64_product": nmod,
+            "speedup_vs_fp64": ms / ims, "err_bound_model": ist["err_bound"],
+            "rel_err_vs_fp64_gpu": ierr, "meets_1e-12": ierr <= 1e-12,
+            "form_ran": ist["precision"], "kernel": ist["gemm_kernel"],
+            "roofline": {"bound": "mfma",
+                         # the GEMM stage is the int8 GEMM and the
+                         # reconstruction of C
+                         "achieved": F * nmod / (igemm * 1e-3) / 1e12,
+                         "peak": INT8_PEAK, "unit": "TOP/s (int8 MFMA)",
+                         "frac": F * nmod / (igemm * 1e-3) / 1e12 / INT8_PEAK,
+                         "kernel": "k_gemm_crt", "kernel_ms": igemm}}
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(

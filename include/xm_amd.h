@@ -28,6 +28,21 @@ extern "C" {
 typedef enum {
 	XM_PRECISION_NATIVE = 0,
 	XM_PRECISION_SPLIT_BF16 = 1,
+	/*
+	 * SPLIT_INT8 - FP64 emulated through the Chinese remainder theorem:
+	 *              each operand row is scaled by a power of two (from its
+	 *              2-norm) and rounded to an integer, the integer product
+	 *              is computed modulo n coprime moduli <= 256 on the int8
+	 *              matrix cores (exact int32 sums over the whole K), and
+	 *              rebuilt in FP64.  Relative error ~0.6 sqrt(K)
+	 *              sqrt(4 W / M) whatever the spread of the entries
+	 *              (M the product of the moduli, W the largest merged pair
+	 *              factor ratio).  Needs the merged pair factors of every C
+	 *              block to be integer multiples of each other (libxm's
+	 *              +-1 / +-2 from antisymmetric blocks are); other
+	 *              contractions run SPLIT_BF16 (the stats say which).
+	 */
+	XM_PRECISION_SPLIT_INT8 = 2,
 } xm_precision_t;
 
 void xm_set_contract_precision(xm_precision_t precision);
@@ -35,6 +50,10 @@ xm_precision_t xm_get_contract_precision(void);
 /* Number of BF16 slices per FP64 operand in SPLIT_BF16 mode (3..8). */
 void xm_set_split_slices(int nslices);
 int xm_get_split_slices(void);
+/* Number of moduli in SPLIT_INT8 mode: 8..13, or 0 (the default) for the
+ * fewest whose modelled error is below 1e-13 for the plan's longest K. */
+void xm_set_split_moduli(int nmoduli);
+int xm_get_split_moduli(void);
 
 /*
  * Graph launch (no libxm counterpart).  Once a contraction plan's GEMM tile
@@ -92,6 +111,11 @@ typedef struct {
 	 * (C blocks at the end of the launch order), summed by a second
 	 * kernel */
 	size_t ksplit_tiles;
+	/* SPLIT_INT8: the modelled relative error of the result (0 in the
+	 * other forms; nslices holds the moduli count) */
+	double err_bound;
+	/* the grouped GEMM kernel the call ran */
+	char gemm_kernel[32];
 } xm_contract_stats_t;
 
 void xm_contract_get_stats(xm_contract_stats_t *stats);

@@ -35,7 +35,8 @@ static int g_nslices = 5;
 XM_API void
 xm_set_contract_precision(xm_precision_t p)
 {
-	if (p != XM_PRECISION_NATIVE && p != XM_PRECISION_SPLIT_BF16)
+	if (p != XM_PRECISION_NATIVE && p != XM_PRECISION_SPLIT_BF16 &&
+	    p != XM_PRECISION_SPLIT_INT8)
 		XM_FATAL("xm_set_contract_precision", "unknown precision");
 	g_precision = p;
 }
@@ -335,6 +336,9 @@ pcontig_panels(const contract_plan *pl)
 		const char *e = getenv("XM_PANEL_PCONTIG");
 		on = !(e && *e == '0');
 	}
+	/* the CRT form's residue kernels read P-contiguous panels only */
+	if (pl->precision == XM_PRECISION_SPLIT_INT8)
+		return true;
 	return on && pl->precision == XM_PRECISION_NATIVE &&
 	    (pl->type != XM_SCALAR_DOUBLE || pl->even_extents);
 }
@@ -709,6 +713,7 @@ plan_release(contract_plan *pl)
 		for (auto &e : pl->ev)
 			HIP_OK("xm_contract", hipEventDestroy(e));
 	split_release(pl);
+	crt_release(pl);
 	delete pl;
 }
 
@@ -958,7 +963,8 @@ get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 		prec = XM_PRECISION_NATIVE;
 	plan_key key{a->id, a->version, b->id, b->version, c->id, c->version,
 	    ia, ib, ic, prec, prec == XM_PRECISION_SPLIT_BF16 ?
-	    current_nslices() : 0};
+	    current_nslices() : prec == XM_PRECISION_SPLIT_INT8 ?
+	    100 * current_nslices() + current_nmoduli() : 0};
 	std::lock_guard<std::mutex> g(cache_mu);
 	for (auto it = cache.begin(); it != cache.end(); ++it)
 		if (!(it->first < key) && !(key < it->first)) {
@@ -977,10 +983,18 @@ get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 		return sp;
 	}
 	contract_plan *pl = build_plan(a, b, c, mk, prec);
-	pl->nslices = key.nslices;
+	if (prec == XM_PRECISION_SPLIT_INT8 && !crt_eligible(pl)) {
+		/* factors without an integer chain: the BF16 form */
+		delete pl;
+		prec = XM_PRECISION_SPLIT_BF16;
+		pl = build_plan(a, b, c, mk, prec);
+	}
+	pl->nslices = prec == XM_PRECISION_SPLIT_BF16 ? current_nslices() : 0;
 	plan_upload(pl);
 	if (prec == XM_PRECISION_SPLIT_BF16)
 		split_prepare(pl, a, b, c, mk);
+	if (prec == XM_PRECISION_SPLIT_INT8)
+		crt_prepare(pl, mk, c);
 	cache.push_front({key, pl});
 	while (cache.size() > CACHE_MAX) {
 		plan_release(cache.back().second);
@@ -1162,9 +1176,13 @@ launch_sequence(contract_plan *pl, uint32_t flags, hipStream_t s, bool stamps)
 		    pl->type, pl->type, 0, s);
 	if (do_pairs && pl->precision == XM_PRECISION_SPLIT_BF16)
 		split_slices(pl, s);
+	if (do_pairs && pl->precision == XM_PRECISION_SPLIT_INT8)
+		crt_quantize(pl, s);
 	mark(1);
 	if (pl->precision == XM_PRECISION_SPLIT_BF16)
 		split_gemm(pl, s);
+	else if (pl->precision == XM_PRECISION_SPLIT_INT8)
+		crt_gemm(pl, s);
 	else
 		launch_gemm(pl, s);
 	mark(2);
@@ -1238,6 +1256,9 @@ xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
 	st.tile_tuning = parts[0]->tile_choice < 0;
 	st.graph = pl->last_graph;
 	st.staged_batches = (int)pl->batches.size();
+	st.err_bound = crt_error_bound(pl);
+	snprintf(st.gemm_kernel, sizeof st.gemm_kernel, "%s",
+	    parts[0]->gemm_kernel);
 	for (size_t i = 0; i < parts.size(); i++)
 		gemm_tile_record(parts[i], gms[i]);
 }
@@ -1499,5 +1520,6 @@ xm_contract_noalloc(xm_scalar_t alpha, const xm_tensor_t *a,
 	st.gemm_launches = pl->gemm_launches;
 	st.tile_p = (int)pl->cand_bp[0];
 	st.tile_q = (int)pl->cand_bq[0];
+	snprintf(st.gemm_kernel, sizeof st.gemm_kernel, "%s", pl->gemm_kernel);
 	return 0;
 }

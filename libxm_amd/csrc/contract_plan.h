@@ -30,8 +30,9 @@ struct plan_pair {
 	uint64_t m, n, k;
 };
 
-/* Split-precision stage state (split.hip). */
+/* Split-precision stage state (split.hip, crt.hip). */
 struct split_state;
+struct crt_state;
 
 struct contract_plan {
 	xm_scalar_type_t type;
@@ -119,6 +120,8 @@ struct contract_plan {
 	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
 	split_state *split = nullptr;
+	crt_state *crt = nullptr;
+	const char *gemm_kernel = "";	/* kernel of the last GEMM launch */
 
 	/* the device launch sequence captured as a graph, one per (alpha == 0,
 	 * beta == 0) combination; graph_failed after a capture error */
@@ -152,6 +155,15 @@ void split_prepare(contract_plan *pl, const xm_tensor_t *a,
 void split_slices(contract_plan *pl, hipStream_t s);
 void split_gemm(contract_plan *pl, hipStream_t s);
 void split_release(contract_plan *pl);
+/* crt.hip */
+bool crt_eligible(const contract_plan *pl);
+void crt_prepare(contract_plan *pl, const masks_t &mk, const xm_tensor_t *c);
+void crt_quantize(contract_plan *pl, hipStream_t s);
+void crt_gemm(contract_plan *pl, hipStream_t s);
+void crt_release(contract_plan *pl);
+double crt_error_bound(const contract_plan *pl);
+int current_nmoduli(void);
+
 /* Whether a split contraction with this many slices runs the LDS-staged
  * kernel (128 x 128 tiles; slices 3-6) rather than the per-wave one (64 x 64
  * tiles; slices 7-8, or any count with XM_SPLIT_KERNEL=1). */

@@ -899,6 +899,8 @@ tile_candidates(const contract_plan *pl, std::vector<std::pair<uint32_t,
 	if (pl->precision == XM_PRECISION_SPLIT_BF16) {
 		out.push_back(split_staged_kernel(pl->nslices) ?
 		    std::make_pair(128u, 128u) : std::make_pair(64u, 64u));
+	} else if (pl->precision == XM_PRECISION_SPLIT_INT8) {
+		out.push_back({256, 128});
 	} else if (pl->type == XM_SCALAR_DOUBLE && !use_simple()) {
 		const char *e = getenv("XM_GEMM_TILE");
 		if (e && strcmp(e, "64x64") == 0)
@@ -1235,6 +1237,26 @@ gemm_tile_record(contract_plan *pl, float gemm_ms)
 	select_tiles(pl);
 }
 
+/* The kernel launch_gemm runs for the plan's active tile shape (the same
+ * choices as below). */
+static const char *
+gemm_kernel_name(const contract_plan *pl)
+{
+	const bool q128 = pl->cand_bq[pl->tile_active] == 128;
+	if (use_simple())
+		return "k_gemm_simple";
+	switch (pl->type) {
+	case XM_SCALAR_DOUBLE:
+		return q128 && pl->f64_dma ? "k_gemm_f64_dma" : "k_gemm_f64";
+	case XM_SCALAR_FLOAT:
+		if (pl->mfma_x4)
+			return "k_gemm_mfma_dw<x4>";
+		return pl->mfma_dw ? "k_gemm_mfma_dw" : "k_gemm_simple";
+	default:
+		return pl->mfma_dw ? "k_gemm_mfma_dw" : "k_gemm_simple";
+	}
+}
+
 void
 launch_gemm(contract_plan *pl, hipStream_t s)
 {
@@ -1309,4 +1331,5 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 		xm_fatal("xm_contract: GEMM launch failed: %s",
 		    hipGetErrorString(e));
 	pl->gemm_launches = 1;
+	pl->gemm_kernel = gemm_kernel_name(pl);
 }

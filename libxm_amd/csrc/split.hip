@@ -777,6 +777,8 @@ split_gemm(contract_plan *pl, hipStream_t s)
 		xm_fatal("xm_contract: split GEMM launch failed: %s",
 		    hipGetErrorString(e));
 	pl->gemm_launches = 1;
+	pl->gemm_kernel = split_staged_kernel(ss->nslices) ? "k_gemm_split2" :
+	    "k_gemm_split";
 }
 
 void

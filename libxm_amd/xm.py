@@ -27,6 +27,7 @@ BLOCK_DERIVATIVE = 2
 
 PRECISION_NATIVE = 0
 PRECISION_SPLIT_BF16 = 1
+PRECISION_SPLIT_INT8 = 2
 
 NULL_PTR = (1 << 64) - 1
 MAX_DIM = 8
@@ -68,7 +69,8 @@ class ContractStats(C.Structure):
                 ("permute_tiled", C.c_size_t),
                 ("permute_gather", C.c_size_t),
                 ("staged_batches", C.c_int), ("staged_bytes", C.c_double),
-                ("ksplit_tiles", C.c_size_t)]
+                ("ksplit_tiles", C.c_size_t), ("err_bound", C.c_double),
+                ("gemm_kernel", C.c_char * 32)]
 
 
 def dim(seq):
@@ -185,6 +187,8 @@ _SIGNATURES = {
     "xm_get_contract_precision": (C.c_int, []),
     "xm_set_split_slices": (None, [C.c_int]),
     "xm_get_split_slices": (C.c_int, []),
+    "xm_set_split_moduli": (None, [C.c_int]),
+    "xm_get_split_moduli": (C.c_int, []),
     "xm_set_graph_launch": (None, [C.c_int]),
     "xm_get_graph_launch": (C.c_int, []),
     "xm_contract_backward": (None, [Scalar, _P, _P, _P, _P, _P, _S, _S, _S]),
@@ -203,6 +207,7 @@ CORE_SYMBOLS = [n for n in _SIGNATURES
                               "xm_set_contract_precision",
                               "xm_get_contract_precision",
                               "xm_set_split_slices", "xm_get_split_slices",
+                              "xm_set_split_moduli", "xm_get_split_moduli",
                               "xm_set_graph_launch", "xm_get_graph_launch",
                               "xm_contract_backward",
                               "xm_contract_noalloc")]
@@ -388,12 +393,18 @@ class XmLib:
     def stats(self):
         s = ContractStats()
         self.lib.xm_contract_get_stats(C.byref(s))
-        return {name: getattr(s, name) for name, _ in ContractStats._fields_}
+        out = {name: getattr(s, name) for name, _ in ContractStats._fields_}
+        out["gemm_kernel"] = out["gemm_kernel"].decode()
+        return out
 
-    def set_precision(self, precision, nslices=None):
+    def set_precision(self, precision, nslices=None, nmoduli=None):
+        """nslices: BF16 slices (SPLIT_BF16); nmoduli: moduli (SPLIT_INT8,
+        0 = chosen per plan)."""
         self.lib.xm_set_contract_precision(int(precision))
         if nslices is not None:
             self.lib.xm_set_split_slices(int(nslices))
+        if nmoduli is not None:
+            self.lib.xm_set_split_moduli(int(nmoduli))
 
     def set_graph_launch(self, on):
         self.lib.xm_set_graph_launch(1 if on else 0)

@@ -1,0 +1,113 @@
+// CPU check of the CRT arithmetic in libxm_amd/csrc/crt_math.h against
+// exact 128-bit integer arithmetic: limb split and residues of integers up
+// to 2^50, and int32 residue-product sums (what the int8 matrix cores
+// compute per modulus) reconstructed by Garner's algorithm.
+#include "crt_math.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+
+static constexpr crt_tables T = crt_make_tables();
+
+static int
+exact_mod(__int128 v, int m)
+{
+	__int128 r = v % m;
+	if (r < 0)
+		r += m;
+	return (int)(r >= (m + 1) / 2 ? r - m : r);
+}
+
+int
+main(void)
+{
+	std::mt19937_64 rng(12345);
+	long bad = 0;
+	// tables
+	for (int i = 0; i < CRT_MAXN; i++) {
+		__int128 p = 1;
+		for (int j = 0; j < i; j++)
+			p *= CRT_MOD[j];
+		if (i && exact_mod(p * T.ginv[i], CRT_MOD[i]) != 1)
+			bad++, printf("ginv %d\n", i);
+		for (int j = 0; j < i; j++) {
+			__int128 q = 1;
+			for (int k = 0; k < j; k++)
+				q *= CRT_MOD[k];
+			if (exact_mod(q, CRT_MOD[i]) != T.pm[i][j])
+				bad++, printf("pm %d %d\n", i, j);
+		}
+	}
+	// the modular reduction on the whole int32 range: both ends, a dense
+	// window around 0 and random values
+	for (int i = 0; i < CRT_MAXN; i++) {
+		const int m = CRT_MOD[i];
+		auto one = [&](int v) {
+			int r = crt_smod(v, m, T.magic[i], T.c31[i], T.hi[i]);
+			if (r != exact_mod(v, m))
+				bad++;
+		};
+		for (long long v = -(1ll << 31); v < -(1ll << 31) + 100000; v++)
+			one((int)v);
+		for (long long v = (1ll << 31) - 100000; v < (1ll << 31); v++)
+			one((int)v);
+		for (int v = -(1 << 20); v <= (1 << 20); v++)
+			one(v);
+		for (int it = 0; it < 2000000; it++)
+			one((int)(uint32_t)rng());
+	}
+	// residues of single integers, |x| < 2^50
+	for (int it = 0; it < 200000; it++) {
+		long long x = (long long)(rng() >> 13) - (1ll << 50);
+		if (it < 8)
+			x = it < 4 ? (1ll << 50) - it : -(1ll << 50) + it;
+		int h, md, lo;
+		crt_limbs((double)x, &h, &md, &lo);
+		if ((__int128)h * (1ll << 34) + (__int128)md * (1 << 17) + lo != x)
+			bad++;
+		for (int i = 0; i < CRT_MAXN; i++) {
+			int r = crt_residue(h, md, lo, T.p34[i], T.p17[i],
+			    CRT_MOD[i], T.magic[i], T.c31[i], T.hi[i]);
+			if (r != exact_mod(x, CRT_MOD[i]) || r < -128 || r > 127)
+				bad++;
+		}
+	}
+	// dot products of K integers of up to 2^49.5: int32 sums of residue
+	// products per modulus, then Garner
+	for (int n = 11; n <= CRT_MAXN; n++) {
+		double maxrel = 0;
+		for (int it = 0; it < 300; it++) {
+			const int K = 1 + (int)(rng() % 4000);
+			const double lim = std::ldexp(1.0, (int)(crt_log2_modulus(n) / 2) - 1) /
+			    std::sqrt((double)K);
+			__int128 exact = 0;
+			int acc[CRT_MAXN] = {0};
+			for (int k = 0; k < K; k++) {
+				std::uniform_real_distribution<double> u(-lim, lim);
+				long long a = (long long)std::rint(u(rng));
+				long long b = (long long)std::rint(u(rng));
+				exact += (__int128)a * b;
+				for (int i = 0; i < n; i++)
+					acc[i] += exact_mod(a, CRT_MOD[i]) *
+					    exact_mod(b, CRT_MOD[i]);
+			}
+			int r[CRT_MAXN];
+			for (int i = 0; i < n; i++)
+				r[i] = crt_smod(acc[i], CRT_MOD[i], T.magic[i],
+				    T.c31[i], T.hi[i]);
+			double got = n == 11 ? crt_reconstruct<11>(r, T) :
+			    n == 12 ? crt_reconstruct<12>(r, T) :
+			    crt_reconstruct<13>(r, T);
+			double want = (double)exact;
+			double rel = want != 0 ? std::fabs(got - want) / std::fabs(want) : std::fabs(got);
+			if (rel > maxrel)
+				maxrel = rel;
+			if (rel > 1e-15)
+				bad++;
+		}
+		printf("n=%d log2M=%.1f max rel %.2e\n", n, crt_log2_modulus(n), maxrel);
+	}
+	printf(bad ? "FAIL %ld\n" : "OK\n", bad);
+	return bad != 0;
+}

@@ -58,7 +58,8 @@ def test_precision_ladder_o64_v512(gpulib, oracle):
     work = w.c.build(gpulib, alloc)
     for name, prec, ns in [("fp64", xm.PRECISION_NATIVE, None),
                            ("split6", xm.PRECISION_SPLIT_BF16, 6),
-                           ("split7", xm.PRECISION_SPLIT_BF16, 7)]:
+                           ("split7", xm.PRECISION_SPLIT_BF16, 7),
+                           ("int8", xm.PRECISION_SPLIT_INT8, None)]:
         gpulib.copy(work, 1.0, c, "abcd", "abcd")
         gpulib.set_precision(prec, ns)
         gpulib.contract(w.alpha, t, v, w.beta, work, *w.idx)
@@ -77,6 +78,7 @@ def test_precision_ladder_o64_v512(gpulib, oracle):
     assert errs["fp64"] <= 1e-13
     assert errs["split6"] <= 1e-12
     assert errs["split7"] <= 1e-12
+    assert errs["int8"] <= 1e-12      # default settings: moduli per plan
     assert errs["fp32"] <= 1e-5
     for x in (t, v, c, work, tf, vf, cf):
         gpulib.tensor_free(x)
@@ -218,9 +220,12 @@ def test_ladder_o64_v384_forward(gpulib, oracle):
                  w.alpha, 0.0, w.idx, pairs)
     errs = {}
     for name, prec, ns in [("fp64", xm.PRECISION_NATIVE, None),
-                           ("split5", xm.PRECISION_SPLIT_BF16, 5)]:
+                           ("split5", xm.PRECISION_SPLIT_BF16, 5),
+                           ("int8", xm.PRECISION_SPLIT_INT8, None)]:
         gpulib.set_precision(prec, ns)
         gpulib.contract(w.alpha, t, v, 0.0, c, *w.idx)
+        st = gpulib.stats()
+        assert st["precision"] == prec
         errs[name] = err(read_blocks(gpulib, c, chosen), ref.out)
     gpulib.set_precision(xm.PRECISION_NATIVE)
     ref.free()
@@ -229,6 +234,7 @@ def test_ladder_o64_v384_forward(gpulib, oracle):
                             fsum / 1e9, chosen[0]), errs)
     assert errs["fp64"] <= 1e-13
     assert errs["split5"] <= 1e-12
+    assert errs["int8"] <= 1e-12
     for x in (t, v, c):
         gpulib.tensor_free(x)
     gpulib.allocator_destroy(alloc)
@@ -298,7 +304,9 @@ def test_backward_ladder_o64_v384_vs_oracle(gpulib, oracle):
                              members, idx)
         refs.append({s: fold_block(spec, s, fs[s], g) for s in srcs})
     for name, prec, ns, tol in (("fp64", xm.PRECISION_NATIVE, None, 1e-13),
-                                ("split5", xm.PRECISION_SPLIT_BF16, 5, 1e-12)):
+                                ("split5", xm.PRECISION_SPLIT_BF16, 5, 1e-12),
+                                ("int8", xm.PRECISION_SPLIT_INT8, None,
+                                 1e-12)):
         gpulib.set_precision(prec, ns)
         dt, dv = w.t.build(gpulib, alloc), w.v.build(gpulib, alloc)
         gpulib.contract_backward(w.alpha, t, v, dc, dt, dv, *w.idx)
