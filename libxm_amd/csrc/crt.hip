@@ -236,7 +236,7 @@ k_crt_quant(const crt_inst *__restrict__ insts, uint32_t ninst,
 #pragma unroll
 		for (int j = 0; j < 16; j++) {
 			const int r = crt_residue(h[j], md[j], lo[j], CT.p34[l],
-			    CT.p17[l], CRT_MOD[l], CT.magic[l], CT.c31[l],
+			    CT.p17[l], CRT_MOD[l], CT.finv[l], CT.lo[l],
 			    CT.hi[l]);
 			w[j / 4] |= (uint32_t)(r & 0xff) << (8 * (j % 4));
 		}

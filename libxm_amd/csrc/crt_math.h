@@ -22,6 +22,14 @@
 
 #define CRT_MAXN 13
 
+/* Products of operands below 2^23 in magnitude: the full-rate 24-bit
+ * multiply on the device (32-bit ones run at a quarter of the rate). */
+#ifdef __HIP_DEVICE_COMPILE__
+#define CRT_MUL24(a, b) __mul24((a), (b))
+#else
+#define CRT_MUL24(a, b) ((a) * (b))
+#endif
+
 /* 256 = 2^8, 255 = 3*5*17, 253 = 11*23, 247 = 13*19, 217 = 7*31; the others
  * are primes */
 static constexpr int CRT_MOD[CRT_MAXN] = {256, 255, 253, 251, 247, 241, 239,
@@ -30,6 +38,7 @@ static constexpr int CRT_MOD[CRT_MAXN] = {256, 255, 253, 251, 247, 241, 239,
 struct crt_tables {
 	int lo[CRT_MAXN], hi[CRT_MAXN];	/* symmetric residue range */
 	uint32_t magic[CRT_MAXN];	/* ceil(2^32 / m) */
+	float finv[CRT_MAXN];		/* 1 / m, rounded to FP32 */
 	int c31[CRT_MAXN];		/* 2^31 mod m, in [0, m) */
 	int p17[CRT_MAXN], p34[CRT_MAXN];	/* 2^17, 2^34 mod m (symmetric) */
 	/* Garner: ginv[i] = (m_0 ... m_{i-1})^-1 mod m_i and
@@ -78,6 +87,7 @@ crt_make_tables(void)
 		t.lo[i] = -(m / 2);
 		t.hi[i] = (m + 1) / 2 - 1;
 		t.magic[i] = (uint32_t)((0x100000000ull + m - 1) / m);
+		t.finv[i] = (float)(1.0 / m);
 		t.c31[i] = (int)(0x80000000ull % m);
 		t.p17[i] = crt_sym(crt_pow2_mod(17, m), m);
 		t.p34[i] = crt_sym(crt_pow2_mod(34, m), m);
@@ -124,6 +134,21 @@ crt_smod(int v, int m, uint32_t magic, int c31, int hi)
 }
 
 /*
+ * v mod m in the symmetric range for |v| < 2^25, at full rate: (float)v is
+ * within 1 of v, so the FP32 quotient rint(v / m) is within 0.03 of the
+ * true one and v - q m lies within 0.53 m of 0; one correction either way.
+ * (The integer form above needs quarter-rate 32-bit multiplies.)
+ */
+CRT_HD static inline int
+crt_smod25(int v, int m, float finv, int lo, int hi)
+{
+	const int q = (int)rintf((float)v * finv);
+	int r = v - CRT_MUL24(q, m);
+	r = r > hi ? r - m : r;
+	return r < lo ? r + m : r;
+}
+
+/*
  * An integer-valued double x (|x| < 2^51) as three int32 limbs:
  * x = h * 2^34 + md * 2^17 + lo with |h| <= 2^17, |md|, |lo| <= 2^16.
  * Every step is exact in FP64.
@@ -142,10 +167,11 @@ crt_limbs(double x, int *h, int *md, int *lo)
 
 /* Residue of limbs (h, md, lo) modulo m_i: |h p34 + md p17 + lo| < 2^25. */
 CRT_HD static inline int
-crt_residue(int h, int md, int lo, int p34, int p17, int m, uint32_t magic,
-    int c31, int hi)
+crt_residue(int h, int md, int lo, int p34, int p17, int m, float finv,
+    int rlo, int rhi)
 {
-	return crt_smod(h * p34 + md * p17 + lo, m, magic, c31, hi);
+	return crt_smod25(CRT_MUL24(h, p34) + CRT_MUL24(md, p17) + lo, m, finv,
+	    rlo, rhi);
 }
 
 /*
@@ -168,10 +194,11 @@ crt_reconstruct(const int *r, const crt_tables &t)
 		int s = 0;	/* (d_0 + m_0 d_1 + ...) mod m_i, unreduced */
 #pragma unroll
 		for (int j = 0; j < i; j++)
-			s += d[j] * t.pm[i][j];
-		s = crt_smod(s, m, t.magic[i], t.c31[i], t.hi[i]);
-		d[i] = crt_smod((r[i] - s) * t.ginv[i], m, t.magic[i],
-		    t.c31[i], t.hi[i]);
+			s += CRT_MUL24(d[j], t.pm[i][j]);
+		/* |s| < 12 * 128 * 128, |(r - s) ginv| < 2^16 */
+		s = crt_smod25(s, m, t.finv[i], t.lo[i], t.hi[i]);
+		d[i] = crt_smod25(CRT_MUL24(r[i] - s, t.ginv[i]), m, t.finv[i],
+		    t.lo[i], t.hi[i]);
 	}
 	double x = d[N - 1];
 #pragma unroll

@@ -56,6 +56,19 @@ main(void)
 			one(v);
 		for (int it = 0; it < 2000000; it++)
 			one((int)(uint32_t)rng());
+		/* the FP32 form on its range |v| < 2^25: every value below 2^21,
+		 * both ends and random values */
+		auto small = [&](int v) {
+			int r = crt_smod25(v, m, T.finv[i], T.lo[i], T.hi[i]);
+			if (r != exact_mod(v, m))
+				bad++;
+		};
+		for (int v = -(1 << 21); v <= (1 << 21); v++)
+			small(v);
+		for (int v = (1 << 25) - 200000; v < (1 << 25); v++)
+			small(v), small(-v);
+		for (int it = 0; it < 2000000; it++)
+			small((int)(rng() % (1u << 26)) - (1 << 25) + 1);
 	}
 	// residues of single integers, |x| < 2^50
 	for (int it = 0; it < 200000; it++) {
@@ -68,7 +81,7 @@ main(void)
 			bad++;
 		for (int i = 0; i < CRT_MAXN; i++) {
 			int r = crt_residue(h, md, lo, T.p34[i], T.p17[i],
-			    CRT_MOD[i], T.magic[i], T.c31[i], T.hi[i]);
+			    CRT_MOD[i], T.finv[i], T.lo[i], T.hi[i]);
 			if (r != exact_mod(x, CRT_MOD[i]) || r < -128 || r > 127)
 				bad++;
 		}
