@@ -264,9 +264,11 @@ k_crt_quant(const crt_inst *__restrict__ insts, uint32_t ninst,
  * prefetch crosses pair and modulus boundaries; the MFMAs walk it as
  * nested loops (the accumulators stay put in registers): where the factor
  * changes between pairs they are multiplied by the ratio (mod m), at the
- * end of each modulus reduced to residues and stored as bytes.  (Wave
- * tiles of 128 x 128 on 256 x 256 tiles, one wave a SIMD, ran slower: the
- * barrier and LDS latencies had no second wave to hide behind.)
+ * end of each modulus reduced to residues and stored as bytes.  On the
+ * o=64 v=384 ladder this ran 10.5 ms; 64 x 128 wave tiles (on 256 x 256
+ * tiles, or in two 4-wave workgroups a CU) ran 10.7 and 11.3 ms, 128 x 128
+ * wave tiles (one wave a SIMD) 12.7 ms: the barrier and LDS latencies had
+ * no second wave to hide behind.
  */
 #define CR_BP 256
 #define CR_BQ 128
