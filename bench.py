@@ -538,8 +538,9 @@ def cpu_permute(sp):
 
 def backward_mode(lib, args):
     """xm_contract_backward of the o=64 v=384 ladder: dt and dv from dc (c's
-    structure, U[-1,1]); FP64 and split (5 slices); TFLOP/s of the two
-    adjoint contractions' F together, wall time per call."""
+    structure, U[-1,1]); FP64, split BF16 (5 slices) and int8 CRT; TFLOP/s
+    of the two adjoint contractions' F together (each canonical block's
+    images merged into its own GEMM terms), wall time per call."""
     from libxm_amd import xm, workloads
     w = workloads.ladder_d2h(args.o, args.v, args.seed)
     alloc, t, v, c = build_workload(lib, w, args.seed)
@@ -548,8 +549,9 @@ def backward_mode(lib, args):
     dt, dv = w.t.build(lib, alloc), w.v.build(lib, alloc)
     out = {"mode": "backward", "workload": w.name, "forms": {}}
     for name, prec, ns in (("fp64", xm.PRECISION_NATIVE, None),
-                           ("split5", xm.PRECISION_SPLIT_BF16, 5)):
-        lib.set_precision(prec, ns)
+                           ("split5", xm.PRECISION_SPLIT_BF16, 5),
+                           ("int8", xm.PRECISION_SPLIT_INT8, None)):
+        lib.set_precision(prec, ns, 0)
         call = lambda: lib.contract_backward(w.alpha, t, v, dc, dt, dv,
                                              *w.idx)
         for _ in range(max(args.warmup, 3)):    # plans and tile tuning

@@ -9,18 +9,24 @@
  *
  *   G_a(X) = alpha * sum_{canonical C, k} dc * b     (one GEMM per pair),
  *   da[S](permute(e, p_X)) += s_X * G_a(X)(e)        for every X with
- *                                                     source S.
+ *                                                     source S,
  *
- * G_a is one more contraction through the same planner and grouped GEMM:
- * xm_contract(alpha, dc', b, 0, G_a, idxc, idxb, idxa), where dc' views the
- * canonical blocks of dc (its derivative blocks count as zero) and G_a has
- * the nonzero blocks of a as independent canonical blocks.  Canonical
- * blocks of G_a are da's own storage; derivative blocks of a get scratch
- * storage and are then folded into their sources, each source block in
- * one pass that reads the gradients of all its derivative blocks (up to
- * FOLD_MAXSRC per launch) through their permutations.
- * db is computed the same way from xm_contract(alpha, a, dc', 0, G_b,
- * idxa, idxc, idxb).
+ * where dc' views the canonical blocks of dc (its derivative blocks count
+ * as zero).  When every derivative block of a permutes the dims da takes
+ * from dc among themselves and those it takes from b among themselves
+ * (antisymmetric index pairs do), da is one contraction into da's
+ * canonical blocks whose block S takes the terms of every image X of S
+ * read in S's index order, identical terms merged (contract_gradient):
+ * on an antisymmetric ladder the images' terms are the canonical block's
+ * own, so each GEMM runs once with factor 2.  Otherwise G_a is a
+ * contraction xm_contract(alpha, dc', b, 0, G_a, idxc, idxb, idxa) into a
+ * tensor with the nonzero blocks of a as independent canonical blocks
+ * (canonical ones on da's storage, derivative ones on scratch), and the
+ * derivative blocks' gradients are folded into their sources, each source
+ * block in one pass that reads the gradients of all its derivative blocks
+ * (up to FOLD_MAXSRC per launch) through their permutations.
+ * db is computed the same way from the contraction of a with dc' (idxa,
+ * idxc -> idxb).
  */
 #include "contract_plan.h"
 
@@ -44,6 +50,33 @@ canonical_view(const xm_tensor_t *dc)
 	return v;
 }
 
+/* dx fits x: same scalar type and block space, canonical wherever x is. */
+static void
+check_gradient(const xm_tensor_t *x, const xm_tensor_t *dx)
+{
+	if (dx->type != x->type)
+		XM_FATAL(FN, "gradient tensors must have the tensor's scalar type");
+	if (!xm_block_space_eq(dx->bs, x->bs))
+		XM_FATAL(FN, "gradient tensors must have the tensor's block space");
+	for (size_t f = 0; f < x->blocks.size(); f++)
+		if (x->blocks[f].type == XM_BLOCK_TYPE_CANONICAL &&
+		    dx->blocks[f].type != XM_BLOCK_TYPE_CANONICAL)
+			XM_FATAL(FN, "gradient tensors must have canonical "
+			    "blocks where the tensor has them");
+}
+
+/* dx canonical exactly where x is (the merged gradient writes every
+ * canonical block of dx). */
+static bool
+same_canonical(const xm_tensor_t *x, const xm_tensor_t *dx)
+{
+	for (size_t f = 0; f < x->blocks.size(); f++)
+		if ((x->blocks[f].type == XM_BLOCK_TYPE_CANONICAL) !=
+		    (dx->blocks[f].type == XM_BLOCK_TYPE_CANONICAL))
+			return false;
+	return true;
+}
+
 /* The gradient tensor of x: every nonzero block of x made canonical;
  * canonical blocks share dx's storage, derivative ones get scratch
  * (returned in scratch). */
@@ -51,10 +84,6 @@ static xm_tensor_t *
 gradient_tensor(const xm_tensor_t *x, xm_tensor_t *dx,
     std::vector<uint64_t> &scratch)
 {
-	if (dx->type != x->type)
-		XM_FATAL(FN, "gradient tensors must have the tensor's scalar type");
-	if (!xm_block_space_eq(dx->bs, x->bs))
-		XM_FATAL(FN, "gradient tensors must have the tensor's block space");
 	xm_tensor_t *g = xm_tensor_create(x->bs, x->type, x->alloc);
 	for (size_t f = 0; f < x->blocks.size(); f++) {
 		const xm_block &bx = x->blocks[f];
@@ -62,9 +91,6 @@ gradient_tensor(const xm_tensor_t *x, xm_tensor_t *dx,
 			continue;
 		xm_dim_t idx = grid_index(x, f);
 		if (bx.type == XM_BLOCK_TYPE_CANONICAL) {
-			if (dx->blocks[f].type != XM_BLOCK_TYPE_CANONICAL)
-				XM_FATAL(FN, "gradient tensors must have canonical "
-				    "blocks where the tensor has them");
 			xm_tensor_set_canonical_block_raw(g, idx,
 			    dx->blocks[f].data);
 		} else {
@@ -158,12 +184,20 @@ bwd_cache_drop_alloc(xm_allocator_t *a)
 			it = nx;
 		}
 	}
+	/* the scratch in the allocator being destroyed goes with it; the
+	 * other allocator's is released */
 	for (auto &x : gone) {
-		x.sa.clear();
-		x.sb.clear();
+		if (x.aa == a)
+			x.sa.clear();
+		if (x.ab == a)
+			x.sb.clear();
 		aux_release(x);
 	}
 }
+
+static void accumulate(xm_contract_stats_t *acc);
+static void fold_images(const xm_tensor_t *x, xm_tensor_t *dx,
+    const xm_tensor_t *g, std::vector<fold_wave> &fold, bool &built);
 
 /* One gradient: g from gradient_tensor; fold: its fold launches, built
  * on the first call (built). */
@@ -174,6 +208,14 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
     std::vector<fold_wave> &fold, bool &built)
 {
 	xm_contract(alpha, p, q, 0, g, ip, iq, ix);
+	accumulate(acc);
+	fold_images(x, dx, g, fold, built);
+}
+
+/* Add the last contraction's figures to acc. */
+static void
+accumulate(xm_contract_stats_t *acc)
+{
 	xm_contract_stats_t st;
 	xm_contract_get_stats(&st);
 	acc->flops += st.flops;
@@ -196,7 +238,16 @@ gradient(xm_scalar_t alpha, const xm_tensor_t *x, xm_tensor_t *dx,
 	acc->precision = st.precision;
 	acc->nslices = st.nslices;
 	acc->plan_cached = st.plan_cached;
+	acc->err_bound = std::max(acc->err_bound, st.err_bound);
+	memcpy(acc->gemm_kernel, st.gemm_kernel, sizeof st.gemm_kernel);
+}
 
+/* Fold the gradients of x's derivative blocks (in g) into their sources
+ * in dx; the launches are built on the first call (built). */
+static void
+fold_images(const xm_tensor_t *x, xm_tensor_t *dx, const xm_tensor_t *g,
+    std::vector<fold_wave> &fold, bool &built)
+{
 	/* fold derivative blocks into their sources:
 	 * dx[S](permute(e, p)) += s * g[X](e), the X of one source in block
 	 * order; one job per source takes up to FOLD_MAXSRC of them in one
@@ -293,16 +344,14 @@ xm_contract_backward(xm_scalar_t alpha, const xm_tensor_t *a,
 				break;
 			}
 	}
+	if (da)
+		check_gradient(a, da);
+	if (db)
+		check_gradient(b, db);
 	if (aux == nullptr) {
 		cur.dcv = canonical_view(dc);
-		if (da) {
-			cur.ga = gradient_tensor(a, da, cur.sa);
-			cur.aa = a->alloc;
-		}
-		if (db) {
-			cur.gb = gradient_tensor(b, db, cur.sb);
-			cur.ab = b->alloc;
-		}
+		cur.aa = a->alloc;
+		cur.ab = b->alloc;
 		std::list<bwd_aux> gone;
 		{
 			std::lock_guard<std::mutex> g(bwd_mu);
@@ -315,12 +364,30 @@ xm_contract_backward(xm_scalar_t alpha, const xm_tensor_t *a,
 		for (auto &x : gone)
 			aux_release(x);
 	}
-	if (da)
-		gradient(alpha, a, da, aux->ga, aux->dcv, b, idxc, idxb, idxa,
-		    &acc, aux->fa, aux->fa_built);
-	if (db)
-		gradient(alpha, b, db, aux->gb, a, aux->dcv, idxa, idxc, idxb,
-		    &acc, aux->fb, aux->fb_built);
+	/* the merged gradient where the images allow it, else the
+	 * per-image gradient tensor and its folds (built on first use) */
+	if (da) {
+		if (same_canonical(a, da) && contract_gradient(alpha, aux->dcv,
+		    b, da, a, idxc, idxb, idxa)) {
+			accumulate(&acc);
+		} else {
+			if (aux->ga == nullptr)
+				aux->ga = gradient_tensor(a, da, aux->sa);
+			gradient(alpha, a, da, aux->ga, aux->dcv, b, idxc, idxb,
+			    idxa, &acc, aux->fa, aux->fa_built);
+		}
+	}
+	if (db) {
+		if (same_canonical(b, db) && contract_gradient(alpha, a,
+		    aux->dcv, db, b, idxa, idxc, idxb)) {
+			accumulate(&acc);
+		} else {
+			if (aux->gb == nullptr)
+				aux->gb = gradient_tensor(b, db, aux->sb);
+			gradient(alpha, b, db, aux->gb, a, aux->dcv, idxa, idxc,
+			    idxb, &acc, aux->fb, aux->fb_built);
+		}
+	}
 	auto t1 = std::chrono::steady_clock::now();
 	acc.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
 	contract_set_stats(acc);

@@ -437,10 +437,151 @@ struct plan_builder {
  * in pinned host memory are staged (see contract_plan) rather than read in
  * place.
  */
+/*
+ * Derivative blocks of the tensor `images` (block space of c) by source
+ * block: the images of every canonical block of c in the gradient plans
+ * of xm_contract_backward.
+ */
+struct image_map {
+	const xm_tensor_t *x;
+	std::map<size_t, std::vector<size_t>> of;	/* source -> images */
+};
+
+/* The free dims of an operand read for image X of a C block: C position
+ * pos[j'] takes X's dim pinv[pos[j']], which is pos[j] for some j (an
+ * image permutes the C block's dims taken from a among themselves, and
+ * those taken from b); the operand dim matched to it is od[j]. */
+static bool
+image_free_dims(const xm_dim_t &perm, const xm_dim_t &pos,
+    const xm_dim_t &od, xm_dim_t *out)
+{
+	xm_dim_t pinv = perm;
+	for (size_t k = 0; k < perm.n; k++)
+		pinv.i[perm.i[k]] = k;
+	*out = od;
+	for (size_t jp = 0; jp < pos.n; jp++) {
+		const size_t want = pinv.i[pos.i[jp]];
+		size_t j = pos.n;
+		for (size_t t = 0; t < pos.n; t++)
+			if (pos.i[t] == want)
+				j = t;
+		if (j == pos.n)
+			return false;
+		out->i[jp] = od.i[j];
+	}
+	return true;
+}
+
+/* Whether every derivative block of x permutes c's dims from a among
+ * themselves and those from b among themselves (x has c's block space). */
+bool
+images_mergeable(const xm_tensor_t *x, const masks_t &mk)
+{
+	for (const xm_block &bx : x->blocks) {
+		if (bx.type != XM_BLOCK_TYPE_DERIVATIVE)
+			continue;
+		xm_dim_t f;
+		if (!image_free_dims(bx.perm, mk.cidxc, mk.aidxa, &f) ||
+		    !image_free_dims(bx.perm, mk.aidxc, mk.aidxb, &f))
+			return false;
+	}
+	return true;
+}
+
+/*
+ * The GEMM terms of canonical block S (flat fc) of c in a gradient plan:
+ * the pairs of S and of every image X = s_X S(permute(., p_X)) of S in
+ * the tensor the gradient belongs to, X's pairs read in S's index order
+ * (operand free dims reordered by p_X) and scaled by s_X; terms reading
+ * the same data in the same order sum the same products and are merged
+ * (factors added in the element precision, zero sums dropped), as libxm
+ * merges pairs within one C block (libxm src/contract.c:86-115).
+ */
+template <typename Emit>
+static void
+image_terms(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
+    const image_map *images, size_t fc, const masks_t &mk, contract_plan *pl,
+    size_t mult, uint64_t m, uint64_t n, Emit &emit)
+{
+	struct term {
+		size_t fa, fb;
+		xm_dim_t Fa, Fb;
+		xm_scalar_t f;
+		uint64_t k;
+	};
+	std::vector<term> terms;
+	std::map<std::vector<uint64_t>, size_t> index;
+	std::vector<std::pair<size_t, xm_block>> ims;
+	xm_block self;
+	memset(&self, 0, sizeof self);
+	self.perm = xm_dim_identity_permutation(c->bs->dims.n);
+	self.scalar = 1;
+	ims.push_back({fc, self});
+	auto it = images->of.find(fc);
+	if (it != images->of.end())
+		for (size_t fx : it->second)
+			ims.push_back({fx, images->x->blocks[fx]});
+	std::vector<block_pair> all;
+	for (auto &im : ims) {
+		xm_dim_t Fa, Fb;
+		image_free_dims(im.second.perm, mk.cidxc, mk.aidxa, &Fa);
+		image_free_dims(im.second.perm, mk.aidxc, mk.aidxb, &Fb);
+		const xm_dim_t blkx = grid_index(c, im.first);
+		pl->npairs_pre += enumerate_pairs(a, b, blkx, mk, all, nullptr);
+		for (const block_pair &bp : all) {
+			if (!bp.nonzero)
+				continue;
+			xm_dim_t da = block_dims_flat(a, bp.fa);
+			const uint64_t k = xm_dim_dot_mask(&da, &mk.cidxa);
+			pl->flops_pre += 2.0 * mult * m * n * k;
+			if (bp.alpha == xm_scalar_t(0))
+				continue;
+			/* the data and the order each operand is read in */
+			std::vector<uint64_t> key;
+			for (int side = 0; side < 2; side++) {
+				const xm_tensor_t *t = side ? b : a;
+				const size_t f = side ? bp.fb : bp.fa;
+				const xm_dim_t &F = side ? Fb : Fa;
+				const xm_dim_t &K = side ? mk.cidxb : mk.cidxa;
+				uint64_t ls[XM_MAX_DIM];
+				xm_dim_t d;
+				logical_strides(t, f, ls, &d);
+				key.push_back(block_source_ptr(t, grid_index(t, f)));
+				for (size_t q = 0; q < F.n; q++) {
+					key.push_back(d.i[F.i[q]]);
+					key.push_back(ls[F.i[q]]);
+				}
+				for (size_t q = 0; q < K.n; q++) {
+					key.push_back(d.i[K.i[q]]);
+					key.push_back(ls[K.i[q]]);
+				}
+			}
+			const xm_scalar_t f = scalar_mul(im.second.scalar, bp.alpha,
+			    c->type);
+			auto ti = index.find(key);
+			if (ti == index.end()) {
+				index[key] = terms.size();
+				terms.push_back({bp.fa, bp.fb, Fa, Fb, f, k});
+			} else {
+				term &t = terms[ti->second];
+				t.f = scalar_add(t.f, f, c->type);
+			}
+		}
+	}
+	for (const term &t : terms) {
+		if (t.f == xm_scalar_t(0))
+			continue;
+		pl->flops += 2.0 * mult * m * n * t.k;
+		pl->npairs++;
+		pl->pair_list.push_back({fc, t.fa, t.fb, t.f, m, n, t.k});
+		emit(t.fa, t.fb, t.Fa, t.Fb, t.f, t.k);
+	}
+}
+
 static contract_plan *
 build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
     const masks_t &mk, int precision, size_t lo = 0, size_t hi = SIZE_MAX,
-    bool stage = false)
+    bool stage = false, const image_map *images = nullptr)
 {
 	contract_plan *pl = new contract_plan;
 	plan_builder pb;
@@ -553,6 +694,44 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 			cb.c = cptr;
 		}
 
+		/* one GEMM term of this C block: operands fa (free dims Fa,
+		 * in the order of the C block's dims taken from a) and fb */
+		auto emit = [&](size_t fa, size_t fb, const xm_dim_t &Fa,
+		    const xm_dim_t &Fb, xm_scalar_t f, uint64_t k) {
+			gemm_pair gp;
+			memset(&gp, 0, sizeof gp);
+			uint64_t pa, pbp;
+			uint8_t ma, mb;
+			uint32_t lda, ldb;
+			bool wa, wb;
+			pb.operand(a, fa, Fa, mk.cidxa, &pa, &ma, &lda, &wa);
+			pb.operand(b, fb, Fb, mk.cidxb, &pbp, &mb, &ldb, &wb);
+			gp.k = (uint32_t)k;
+			gp.a_re = f.real();
+			gp.a_im = f.imag();
+			if (!swap) {
+				gp.x = pa; gp.xmode = ma; gp.ldx = lda;
+				gp.y = pbp; gp.ymode = mb; gp.ldy = ldb;
+				pl->x_in_ws.push_back(wa);
+				pl->y_in_ws.push_back(wb);
+			} else {
+				gp.x = pbp; gp.xmode = mb; gp.ldx = ldb;
+				gp.y = pa; gp.ymode = ma; gp.ldy = lda;
+				pl->x_in_ws.push_back(wb);
+				pl->y_in_ws.push_back(wa);
+			}
+			pl->pairs.push_back(gp);
+			/* split path: which operand is A (kept for the
+			 * slicing stage) */
+			pl->pair_swap.push_back(swap);
+		};
+		if (images != nullptr) {
+			image_terms(a, b, c, images, fc, mk, pl, mult, m, n, emit);
+			cb.pair_end = (uint32_t)pl->pairs.size();
+			pl->cblocks.push_back(cb);
+			pl->cpanel.push_back(cpanel);
+			continue;
+		}
 		size_t npre = enumerate_pairs(a, b, blkc, mk, all, &members);
 		pl->npairs_pre += npre;
 		for (size_t i = 0; i < all.size(); i++) {
@@ -594,33 +773,7 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 			}
 			if (best.fa != all[i].fa || best.fb != all[i].fb)
 				pl->nsubstituted++;
-
-			gemm_pair gp;
-			memset(&gp, 0, sizeof gp);
-			uint64_t pa, pbp;
-			uint8_t ma, mb;
-			uint32_t lda, ldb;
-			bool wa, wb;
-			pb.operand(a, best.fa, mk.aidxa, mk.cidxa, &pa, &ma, &lda, &wa);
-			pb.operand(b, best.fb, mk.aidxb, mk.cidxb, &pbp, &mb, &ldb, &wb);
-			gp.k = (uint32_t)k;
-			gp.a_re = all[i].alpha.real();
-			gp.a_im = all[i].alpha.imag();
-			if (!swap) {
-				gp.x = pa; gp.xmode = ma; gp.ldx = lda;
-				gp.y = pbp; gp.ymode = mb; gp.ldy = ldb;
-				pl->x_in_ws.push_back(wa);
-				pl->y_in_ws.push_back(wb);
-			} else {
-				gp.x = pbp; gp.xmode = mb; gp.ldx = ldb;
-				gp.y = pa; gp.ymode = ma; gp.ldy = lda;
-				pl->x_in_ws.push_back(wb);
-				pl->y_in_ws.push_back(wa);
-			}
-			pl->pairs.push_back(gp);
-			/* split path: which operand is A (kept for the
-			 * slicing stage) */
-			pl->pair_swap.push_back(swap);
+			emit(best.fa, best.fb, mk.aidxa, mk.aidxb, all[i].alpha, k);
 		}
 		cb.pair_end = (uint32_t)pl->pairs.size();
 		pl->cblocks.push_back(cb);
@@ -671,10 +824,11 @@ struct plan_key {
 	uint64_t ida, va, idb, vb, idc, vc;
 	std::string ia, ib, ic;
 	int precision, nslices;
+	uint64_t idx = 0, vx = 0;	/* gradient plans: the images' tensor */
 	bool operator<(const plan_key &o) const {
 		return std::tie(ida, va, idb, vb, idc, vc, ia, ib, ic, precision,
-		    nslices) < std::tie(o.ida, o.va, o.idb, o.vb, o.idc, o.vc,
-		    o.ia, o.ib, o.ic, o.precision, o.nslices);
+		    nslices, idx, vx) < std::tie(o.ida, o.va, o.idb, o.vb, o.idc,
+		    o.vc, o.ia, o.ib, o.ic, o.precision, o.nslices, o.idx, o.vx);
 	}
 };
 
@@ -956,7 +1110,7 @@ build_staged(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 static contract_plan *
 get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
     const char *ia, const char *ib, const char *ic, const masks_t &mk,
-    bool *cached)
+    bool *cached, const xm_tensor_t *x = nullptr)
 {
 	int prec = current_precision();
 	if (c->type != XM_SCALAR_DOUBLE)
@@ -965,6 +1119,16 @@ get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 	    ia, ib, ic, prec, prec == XM_PRECISION_SPLIT_BF16 ?
 	    current_nslices() : prec == XM_PRECISION_SPLIT_INT8 ?
 	    100 * current_nslices() + current_nmoduli() : 0};
+	image_map images;
+	if (x) {
+		key.idx = x->id;
+		key.vx = x->version;
+		images.x = x;
+		for (size_t f = 0; f < x->blocks.size(); f++)
+			if (x->blocks[f].type == XM_BLOCK_TYPE_DERIVATIVE)
+				images.of[(size_t)x->blocks[f].data].push_back(f);
+	}
+	const image_map *im = x ? &images : nullptr;
 	std::lock_guard<std::mutex> g(cache_mu);
 	for (auto it = cache.begin(); it != cache.end(); ++it)
 		if (!(it->first < key) && !(key < it->first)) {
@@ -973,7 +1137,7 @@ get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 			return cache.front().second;
 		}
 	*cached = false;
-	if (touches_host(a, b, c)) {
+	if (x == nullptr && touches_host(a, b, c)) {
 		contract_plan *sp = build_staged(a, b, c, mk);
 		cache.push_front({key, sp});
 		while (cache.size() > CACHE_MAX) {
@@ -982,12 +1146,13 @@ get_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 		}
 		return sp;
 	}
-	contract_plan *pl = build_plan(a, b, c, mk, prec);
+	contract_plan *pl = build_plan(a, b, c, mk, prec, 0, SIZE_MAX, false,
+	    im);
 	if (prec == XM_PRECISION_SPLIT_INT8 && !crt_eligible(pl)) {
 		/* factors without an integer chain: the BF16 form */
 		delete pl;
 		prec = XM_PRECISION_SPLIT_BF16;
-		pl = build_plan(a, b, c, mk, prec);
+		pl = build_plan(a, b, c, mk, prec, 0, SIZE_MAX, false, im);
 	}
 	pl->nslices = prec == XM_PRECISION_SPLIT_BF16 ? current_nslices() : 0;
 	plan_upload(pl);
@@ -1191,6 +1356,9 @@ launch_sequence(contract_plan *pl, uint32_t flags, hipStream_t s, bool stamps)
 	mark(3);
 }
 
+static void record_stats(contract_plan *pl, bool cached, xm_scalar_t alpha,
+    xm_scalar_t beta, double total_ms);
+
 XM_API void
 xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
     xm_scalar_t beta, xm_tensor_t *c, const char *idxa, const char *idxb,
@@ -1206,6 +1374,41 @@ xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
 	else
 		run_staged(pl, alpha, beta);
 	auto t1 = std::chrono::steady_clock::now();
+	record_stats(pl, cached, alpha, beta,
+	    std::chrono::duration<double, std::milli>(t1 - t0).count());
+}
+
+/*
+ * dx = alpha * p (x) q over dx's canonical blocks, each block S also
+ * taking the terms of x's derivative blocks whose source is S, read in
+ * S's index order and merged with S's own (image_terms): the adjoint's
+ * gradient in one grouped GEMM, without per-image outputs or folds.
+ * Returns false (nothing done) when an image mixes the dims dx takes
+ * from p with those from q, or a block is in host memory.
+ */
+bool
+contract_gradient(xm_scalar_t alpha, const xm_tensor_t *p,
+    const xm_tensor_t *q, xm_tensor_t *dx, const xm_tensor_t *x,
+    const char *ip, const char *iq, const char *ix)
+{
+	auto t0 = std::chrono::steady_clock::now();
+	masks_t mk;
+	contract_masks("xm_contract_backward", p, q, dx, ip, iq, ix, mk);
+	if (!images_mergeable(x, mk) || touches_host(p, q, dx))
+		return false;
+	bool cached = false;
+	contract_plan *pl = get_plan(p, q, dx, ip, iq, ix, mk, &cached, x);
+	run_plan(pl, alpha, 0);
+	auto t1 = std::chrono::steady_clock::now();
+	record_stats(pl, cached, alpha, 0,
+	    std::chrono::duration<double, std::milli>(t1 - t0).count());
+	return true;
+}
+
+static void
+record_stats(contract_plan *pl, bool cached, xm_scalar_t alpha,
+    xm_scalar_t beta, double total_ms)
+{
 	xm_contract_stats_t &st = last_stats;
 	memset(&st, 0, sizeof st);
 	const bool ops = alpha != xm_scalar_t(0), cin = beta != xm_scalar_t(0);
@@ -1247,7 +1450,7 @@ xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
 			st.staged_bytes += q->staged_bytes;
 		gms.push_back((float)ms[1]);
 	}
-	st.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+	st.total_ms = total_ms;
 	st.plan_cached = cached;
 	st.precision = pl->precision;
 	st.nslices = pl->nslices;

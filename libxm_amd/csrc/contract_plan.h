@@ -142,6 +142,10 @@ int current_nslices(void);
 void plan_cache_clear(void);
 void plan_cache_drop(uint64_t tensor_id);
 void contract_set_stats(const xm_contract_stats_t &s);
+bool images_mergeable(const xm_tensor_t *x, const masks_t &mk);
+bool contract_gradient(xm_scalar_t alpha, const xm_tensor_t *p,
+    const xm_tensor_t *q, xm_tensor_t *dx, const xm_tensor_t *x,
+    const char *ip, const char *iq, const char *ix);
 
 /* gemm.hip */
 void build_tiles(contract_plan *pl);
