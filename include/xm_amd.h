@@ -47,7 +47,8 @@ typedef enum {
 
 void xm_set_contract_precision(xm_precision_t precision);
 xm_precision_t xm_get_contract_precision(void);
-/* Number of BF16 slices per FP64 operand in SPLIT_BF16 mode (3..8). */
+/* Number of BF16 slices per FP64 operand in SPLIT_BF16 mode (3..8;
+ * default 6). */
 void xm_set_split_slices(int nslices);
 int xm_get_split_slices(void);
 /* Number of moduli in SPLIT_INT8 mode: 8..13, or 0 (the default) for the

@@ -61,3 +61,18 @@ def ref(recorder):
 def gpulib():
     """The MI355X library.  Fails loudly if it is not built."""
     return xm.load(LIBXM_SO)
+
+
+@pytest.fixture(autouse=True)
+def _restore_precision(request):
+    """Tests that use the MI355X library leave its precision settings as
+    they found them (the library defaults, for the tests that check
+    them)."""
+    if "gpulib" not in request.fixturenames:
+        yield
+        return
+    lib = request.getfixturevalue("gpulib")
+    saved = (lib.lib.xm_get_contract_precision(),
+             lib.lib.xm_get_split_slices(), lib.lib.xm_get_split_moduli())
+    yield
+    lib.set_precision(*saved)

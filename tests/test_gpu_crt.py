@@ -21,8 +21,9 @@ pytestmark = pytest.mark.gpu
 def crt(gpulib):
     def use(nmod=0):
         gpulib.set_precision(xm.PRECISION_SPLIT_INT8, None, nmod)
+    default = gpulib.lib.xm_get_split_slices()
     yield use
-    gpulib.set_precision(xm.PRECISION_NATIVE, 5, 0)
+    gpulib.set_precision(xm.PRECISION_NATIVE, default, 0)
 
 
 def data_for(specs, seed, wide=False, pairs=None):

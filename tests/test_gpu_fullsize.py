@@ -56,9 +56,13 @@ def test_precision_ladder_o64_v512(gpulib, oracle):
                  w.beta, w.idx, pairs)
     errs = {}
     work = w.c.build(gpulib, alloc)
+    default = gpulib.lib.xm_get_split_slices()
+    assert default == 6
     for name, prec, ns in [("fp64", xm.PRECISION_NATIVE, None),
                            ("split6", xm.PRECISION_SPLIT_BF16, 6),
                            ("split7", xm.PRECISION_SPLIT_BF16, 7),
+                           ("split_default", xm.PRECISION_SPLIT_BF16,
+                            default),
                            ("int8", xm.PRECISION_SPLIT_INT8, None)]:
         gpulib.copy(work, 1.0, c, "abcd", "abcd")
         gpulib.set_precision(prec, ns)
@@ -79,6 +83,7 @@ def test_precision_ladder_o64_v512(gpulib, oracle):
     assert errs["split6"] <= 1e-12
     assert errs["split7"] <= 1e-12
     assert errs["int8"] <= 1e-12      # default settings: moduli per plan
+    assert errs["split_default"] <= 1e-12
     assert errs["fp32"] <= 1e-5
     for x in (t, v, c, work, tf, vf, cf):
         gpulib.tensor_free(x)

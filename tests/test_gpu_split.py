@@ -1,6 +1,6 @@
 """Split-precision (BF16 slices) FP64 contraction on the GPU, checked
 against the FP64 CPU oracle: relative Frobenius error <= 1e-12 at the
-default 5 slices, on ladders (operands read in place), the ring (operands
+5 slices, on ladders (operands read in place), the ring (operands
 and C permuted), and data spread over six decades like the precision
 study's inputs (U[-1,1] * 10^U[-6,0])."""
 import numpy as np
@@ -16,10 +16,12 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def split(gpulib):
+    default = gpulib.lib.xm_get_split_slices()
+
     def use(n):
         gpulib.set_precision(xm.PRECISION_SPLIT_BF16, n)
     yield use
-    gpulib.set_precision(xm.PRECISION_NATIVE, 5)
+    gpulib.set_precision(xm.PRECISION_NATIVE, default)
 
 
 def data_for(specs, seed, wide=False, pairs=None):
