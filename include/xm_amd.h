@@ -117,6 +117,9 @@ typedef struct {
 	double err_bound;
 	/* the grouped GEMM kernel the call ran */
 	char gemm_kernel[32];
+	/* bytes of blocks in an allocator's file read (pread into pinned
+	 * buffers, then staged like host blocks) and written back per call */
+	double file_bytes;
 } xm_contract_stats_t;
 
 void xm_contract_get_stats(xm_contract_stats_t *stats);
@@ -193,6 +196,10 @@ void xm_amd_fill_random(xm_tensor_t *t, uint64_t seed, double lo, double hi,
  * arena.  No libxm counterpart. */
 void xm_amd_allocator_usage(xm_allocator_t *allocator, size_t *device_bytes,
     size_t *host_bytes);
+
+/* Bytes of live allocations in the file of an allocator created with a
+ * path (0 without one).  No libxm counterpart. */
+size_t xm_amd_allocator_file_bytes(xm_allocator_t *allocator);
 
 /* Block until all work issued by the library has completed; returns 0.
  * No libxm counterpart (every xm_* operation is already synchronous). */

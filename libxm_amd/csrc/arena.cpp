@@ -1,32 +1,52 @@
 /*
- * arena.cpp - block storage in HBM (replaces libxm's disk pagefile
- * allocator, libxm src/alloc.c:147-347).
+ * arena.cpp - block storage (replaces libxm's pagefile allocator, libxm
+ * src/alloc.c:147-347), in three tiers.
  *
- * Storage is a list of slabs obtained from hipMalloc, carved first-fit
- * into 256-byte aligned allocations.  Slabs grow geometrically (64 MiB up
- * to 8 GiB) so a tensor of many blocks costs few hipMalloc calls and the
- * arena can fill the 288 GB of an MI355X.  data_ptr is the device address
- * of the allocation.  When HBM is exhausted, or the arena reaches the
- * limit given by XM_ARENA_BYTES, new slabs come from pinned host memory
- * that the GPU reads directly over the host link, so a tensor larger than
- * HBM still works (slowly).  All entry points take the allocator's lock.
+ * HBM: slabs obtained from hipMalloc, carved first-fit into 256-byte
+ * aligned allocations.  Slabs grow geometrically (64 MiB up to 8 GiB) so
+ * a tensor of many blocks costs few hipMalloc calls and the arena can fill
+ * the 288 GB of an MI355X.  data_ptr is the device address of the
+ * allocation.
+ * Pinned host: when HBM is exhausted, or the arena reaches the limit
+ * given by XM_ARENA_BYTES, new slabs come from pinned host memory that the
+ * GPU reads directly over the host link, up to XM_HOST_BYTES (default: no
+ * limit without a path, half the physical memory with one).
+ * File: allocators created with a path (libxm's pagefile allocators) put
+ * what does not fit the first two tiers in that file, created at
+ * xm_allocator_create and removed by xm_allocator_destroy.  data_ptr is
+ * then a file pointer (filestore.h) that no kernel dereferences: the
+ * operations read such blocks into pinned buffers and write them back.
+ * All entry points take the allocator's lock.
  */
 #include "internal.h"
 
 #include <algorithm>
 #include <atomic>
+#include <cerrno>
 #include <cstdio>
 #include <cstdlib>
+#include <unistd.h>
 
 static const size_t ALIGN = 256;
 static const size_t SLAB_MIN = 64ull << 20;
 static const size_t SLAB_MAX = 8ull << 30;
 static std::atomic<uint64_t> next_alloc_id{1};
 
+/* Allocators with a backing file, by the slot their file pointers carry. */
+static std::mutex freg_mu;
+static std::vector<xm_allocator_t *> freg;
+
 static size_t
 round_up(size_t x, size_t a)
 {
 	return (x + a - 1) / a * a;
+}
+
+static size_t
+env_bytes(const char *name, size_t dflt)
+{
+	const char *e = getenv(name);
+	return e ? (size_t)strtoull(e, nullptr, 10) : dflt;
 }
 
 XM_API xm_allocator_t *
@@ -40,11 +60,43 @@ xm_allocator_create(const char *path)
 	a->has_path = path != nullptr;
 	if (path)
 		a->path = path;
-	a->device_bytes = a->host_bytes = 0;
-	a->limit_bytes = SIZE_MAX;
+	a->device_bytes = a->host_bytes = a->file_bytes = 0;
+	a->file = nullptr;
+	a->file_slot = 0;
 	a->id = next_alloc_id++;
-	if (const char *lim = getenv("XM_ARENA_BYTES"))
-		a->limit_bytes = strtoull(lim, nullptr, 10);
+	a->limit_bytes = env_bytes("XM_ARENA_BYTES", SIZE_MAX);
+	size_t half = SIZE_MAX;
+	if (path) {
+		long pages = sysconf(_SC_PHYS_PAGES), psz = sysconf(_SC_PAGESIZE);
+		if (pages > 0 && psz > 0)
+			half = (size_t)pages * (size_t)psz / 2;
+	}
+	a->host_limit = env_bytes("XM_HOST_BYTES", half);
+	if (path) {
+		file_store *f = new file_store;
+		if (!f->create(path)) {
+			delete f;
+			delete a;
+			return nullptr;
+		}
+		std::lock_guard<std::mutex> g(freg_mu);
+		size_t slot = 0;
+		while (slot < freg.size() && freg[slot] != nullptr)
+			slot++;
+		if (slot > FILE_SLOT_MAX) {
+			fprintf(stderr, "libxm: xm_allocator_create: more than %u "
+			    "allocators with a path\n", FILE_SLOT_MAX + 1);
+			f->destroy();
+			delete f;
+			delete a;
+			return nullptr;
+		}
+		if (slot == freg.size())
+			freg.push_back(nullptr);
+		freg[slot] = a;
+		a->file = f;
+		a->file_slot = (uint32_t)slot;
+	}
 	return a;
 }
 
@@ -82,6 +134,8 @@ new_slab(xm_allocator_t *a, size_t need)
 	}
 	if (s.base == nullptr) {
 		bytes = round_up(need, 2ull << 20);
+		if (a->host_bytes + bytes > a->host_limit)
+			return false;
 		if (hipHostMalloc((void **)&s.base, bytes,
 		    hipHostMallocMapped) != hipSuccess) {
 			(void)hipGetLastError();
@@ -120,6 +174,9 @@ arena_host_slab(xm_allocator_t *a, uint64_t ptr)
 {
 	if (a == nullptr || ptr == XM_NULL_PTR)
 		return 0;
+	/* a file counts as one slab */
+	if (file_ptr(ptr))
+		return ptr & ~FILE_OFF_MASK;
 	std::lock_guard<std::mutex> g(a->mu);
 	if (a->host_bytes == 0)
 		return 0;
@@ -141,9 +198,17 @@ xm_allocator_allocate(xm_allocator_t *a, size_t size_bytes)
 		if ((p = carve(s, bytes)) != XM_NULL_PTR)
 			break;
 	if (p == XM_NULL_PTR) {
-		if (!new_slab(a, bytes))
+		if (new_slab(a, bytes)) {
+			p = carve(a->slabs.back(), bytes);
+		} else if (a->file) {
+			const uint64_t off = a->file->allocate(bytes);
+			if (off == UINT64_MAX)
+				return XM_NULL_PTR;
+			p = file_make_ptr(a->file_slot, off);
+			a->file_bytes += bytes;
+		} else {
 			return XM_NULL_PTR;
-		p = carve(a->slabs.back(), bytes);
+		}
 	}
 	auto it = std::lower_bound(a->live.begin(), a->live.end(),
 	    std::make_pair(p, (size_t)0));
@@ -174,6 +239,11 @@ xm_allocator_deallocate(xm_allocator_t *a, uint64_t p)
 		XM_FATAL("xm_allocator_deallocate", "unknown data pointer");
 	size_t bytes = it->second;
 	a->live.erase(it);
+	if (file_ptr(p)) {
+		a->file->release(file_off(p), bytes);
+		a->file_bytes -= bytes;
+		return;
+	}
 	arena_slab *s = slab_of(a, p);
 	size_t off = p - (uint64_t)(uintptr_t)s->base;
 	auto &fl = s->free_list;
@@ -211,6 +281,10 @@ XM_API void
 xm_allocator_read(xm_allocator_t *a, uint64_t p, void *mem, size_t bytes)
 {
 	check_range("xm_allocator_read", a, p, bytes);
+	if (file_ptr(p)) {
+		arena_file_read("xm_allocator_read", p, mem, bytes);
+		return;
+	}
 	HIP_OK("xm_allocator_read", hipMemcpyAsync(mem,
 	    (const void *)(uintptr_t)p, bytes, hipMemcpyDeviceToHost,
 	    xm_stream()));
@@ -223,6 +297,10 @@ xm_allocator_write(xm_allocator_t *a, uint64_t p, const void *mem,
 {
 	check_range("xm_allocator_write", a, p, bytes);
 	bump_data_epoch();
+	if (file_ptr(p)) {
+		arena_file_write("xm_allocator_write", p, mem, bytes);
+		return;
+	}
 	HIP_OK("xm_allocator_write", hipMemcpyAsync((void *)(uintptr_t)p, mem,
 	    bytes, hipMemcpyHostToDevice, xm_stream()));
 	HIP_OK("xm_allocator_write", hipStreamSynchronize(xm_stream()));
@@ -245,6 +323,14 @@ xm_allocator_destroy(xm_allocator_t *a)
 		else
 			HIP_OK("xm_allocator_destroy", hipFree(s.base));
 	}
+	if (a->file) {
+		{
+			std::lock_guard<std::mutex> g(freg_mu);
+			freg[a->file_slot] = nullptr;
+		}
+		a->file->destroy();
+		delete a->file;
+	}
 	delete a;
 }
 
@@ -256,4 +342,104 @@ xm_amd_allocator_usage(xm_allocator_t *a, size_t *dev, size_t *host)
 		*dev = a->device_bytes;
 	if (host)
 		*host = a->host_bytes;
+}
+
+/* Bytes of live allocations in the allocator's file.  No libxm
+ * counterpart. */
+XM_API size_t
+xm_amd_allocator_file_bytes(xm_allocator_t *a)
+{
+	std::lock_guard<std::mutex> g(a->mu);
+	return a->file_bytes;
+}
+
+static file_store *
+store_of(const char *fn, uint64_t p)
+{
+	std::lock_guard<std::mutex> g(freg_mu);
+	const uint32_t slot = file_slot(p);
+	if (slot >= freg.size() || freg[slot] == nullptr)
+		XM_FATAL(fn, "file pointer of no open allocator");
+	return freg[slot]->file;
+}
+
+void
+arena_file_read(const char *fn, uint64_t p, void *buf, size_t bytes)
+{
+	file_store *f = store_of(fn, p);
+	errno = 0;
+	if (!f->read(file_off(p), buf, bytes))
+		xm_fatal("%s: pread: %s", fn, errno ? strerror(errno) :
+		    "end of file");
+}
+
+void
+arena_file_write(const char *fn, uint64_t p, const void *buf, size_t bytes)
+{
+	file_store *f = store_of(fn, p);
+	errno = 0;
+	if (!f->write(file_off(p), buf, bytes))
+		xm_fatal("%s: pwrite: %s", fn, errno ? strerror(errno) :
+		    "no bytes written");
+}
+
+bool
+tensor_in_file(const xm_tensor_t *t)
+{
+	if (t == nullptr)
+		return false;
+	for (const xm_block &b : t->blocks)
+		if (b.type == XM_BLOCK_TYPE_CANONICAL && file_ptr(b.data))
+			return true;
+	return false;
+}
+
+/* ---- staging of file ranges for element kernels -------------------- */
+
+size_t
+file_batch_bytes(void)
+{
+	return env_bytes("XM_STAGE_BYTES", 1ull << 30);
+}
+
+void
+file_stage::load(const char *fn)
+{
+	rd.finish();
+	wr.finish();
+	buf = (char *)pinned_buffer(slot, rd.bytes + wr.bytes);
+	for (const auto &x : rd.r)
+		arena_file_read(fn, x.p, buf + x.off, x.n);
+	for (const auto &x : wr.r)
+		if (x.load)
+			arena_file_read(fn, x.p, buf + rd.bytes + x.off, x.n);
+}
+
+uint64_t
+file_stage::src(uint64_t p) const
+{
+	if (!file_ptr(p))
+		return p;
+	uint64_t o = rd.offset(p);
+	if (o == UINT64_MAX)
+		XM_FATAL("libxm", "file range not staged");
+	return (uint64_t)(uintptr_t)(buf + o);
+}
+
+uint64_t
+file_stage::dst(uint64_t p) const
+{
+	if (!file_ptr(p))
+		return p;
+	uint64_t o = wr.offset(p);
+	if (o == UINT64_MAX)
+		XM_FATAL("libxm", "file range not staged");
+	return (uint64_t)(uintptr_t)(buf + rd.bytes + o);
+}
+
+void
+file_stage::store(const char *fn)
+{
+	for (const auto &x : wr.r)
+		arena_file_write(fn, x.p, buf + rd.bytes + x.off, x.n);
 }

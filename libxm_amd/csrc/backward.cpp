@@ -242,6 +242,49 @@ accumulate(xm_contract_stats_t *acc)
 	memcpy(acc->gemm_kernel, st.gemm_kernel, sizeof st.gemm_kernel);
 }
 
+/* One wave of fold jobs over blocks of which some lie in a file: batches
+ * within file_batch_bytes(), the destinations and gradients of each read
+ * into pinned memory and the destinations written back. */
+static void
+fold_file_wave(const std::vector<fold_job> &wv, size_t esz,
+    xm_scalar_type_t type, hipStream_t s)
+{
+	const uint64_t budget = file_batch_bytes();
+	size_t i = 0;
+	while (i < wv.size()) {
+		file_stage fs(1);
+		uint64_t pend = 0;
+		size_t e = i;
+		for (; e < wv.size(); e++) {
+			const fold_job &j = wv[e];
+			const uint64_t one = (j.n * esz + 255) / 256 * 256;
+			uint64_t cost = file_ptr(j.dst) ? one : 0;
+			for (uint32_t m = 0; m < j.nsrc; m++)
+				cost += file_ptr(j.src[m]) ? one : 0;
+			if (e > i && pend + cost > budget)
+				break;
+			pend += cost;
+			fs.need_write(j.dst, j.n * esz, true);
+			for (uint32_t m = 0; m < j.nsrc; m++)
+				fs.need_read(j.src[m], j.n * esz);
+		}
+		fs.load(FN);
+		std::vector<fold_job> b(wv.begin() + i, wv.begin() + e);
+		for (fold_job &j : b) {
+			j.dst = fs.dst(j.dst);
+			for (uint32_t m = 0; m < j.nsrc; m++)
+				j.src[m] = fs.src(j.src[m]);
+		}
+		size_t nchunks = 0;
+		std::vector<char> pk = fold_jobs_pack(b, &nchunks);
+		void *d = upload(5, pk.data(), pk.size());
+		launch_fold_packed(d, b.size(), nchunks, type, s);
+		HIP_OK(FN, hipStreamSynchronize(s));
+		fs.store(FN);
+		i = e;
+	}
+}
+
 /* Fold the gradients of x's derivative blocks (in g) into their sources
  * in dx; the launches are built on the first call (built). */
 static void
@@ -253,7 +296,9 @@ fold_images(const xm_tensor_t *x, xm_tensor_t *dx, const xm_tensor_t *g,
 	 * order; one job per source takes up to FOLD_MAXSRC of them in one
 	 * pass over dx[S] (launch g takes each source's g-th group) */
 	hipStream_t s = xm_stream();
-	if (built) {
+	/* blocks in a file: staged per call, nothing kept */
+	const bool file = tensor_in_file(dx) || tensor_in_file(g);
+	if (built && !file) {
 		for (const fold_wave &f : fold)
 			launch_fold_packed(f.dev, f.njobs, f.nchunks, x->type, s);
 		HIP_OK(FN, hipStreamSynchronize(s));
@@ -302,6 +347,10 @@ fold_images(const xm_tensor_t *x, xm_tensor_t *dx, const xm_tensor_t *g,
 			waves[w].push_back(e.second[w]);
 		}
 	for (auto &wv : waves) {
+		if (file) {
+			fold_file_wave(wv, elem_size(x->type), x->type, s);
+			continue;
+		}
 		fold_wave fw{nullptr, wv.size(), 0};
 		std::vector<char> pk = fold_jobs_pack(wv, &fw.nchunks);
 		HIP_OK(FN, hipMalloc(&fw.dev, pk.size()));
@@ -310,7 +359,8 @@ fold_images(const xm_tensor_t *x, xm_tensor_t *dx, const xm_tensor_t *g,
 		fold.push_back(fw);
 		launch_fold_packed(fw.dev, fw.njobs, fw.nchunks, x->type, s);
 	}
-	built = true;
+	if (!file)
+		built = true;
 	HIP_OK(FN, hipStreamSynchronize(s));
 }
 

@@ -850,6 +850,9 @@ plan_release(contract_plan *pl)
 	for (void *r : pl->region)
 		if (r)
 			HIP_OK("xm_contract", hipFree(r));
+	for (void *r : pl->bounce)
+		if (r)
+			HIP_OK("xm_contract", hipHostFree(r));
 	for (hipEvent_t e : {pl->ev_start, pl->ev_staged[0], pl->ev_staged[1],
 	    pl->ev_done[0], pl->ev_done[1]})
 		if (e)
@@ -968,12 +971,17 @@ plan_upload(contract_plan *pl)
 		HIP_OK(fn, hipEventCreate(&e));
 }
 
-/* Does the contraction read or write any block in pinned host memory?
- * (XM_HOST_STAGING=0: treat host blocks like HBM ones, read in place.) */
+/* Does the contraction read or write any block in pinned host memory or
+ * in a file?  (XM_HOST_STAGING=0: treat host blocks like HBM ones, read
+ * in place; file blocks are staged regardless.) */
 static bool
 touches_host(const xm_tensor_t *a, const xm_tensor_t *b,
     const xm_tensor_t *c)
 {
+	/* blocks in a file are always staged: no kernel reads a file */
+	for (const xm_tensor_t *t : {a, b, c})
+		if (tensor_in_file(t))
+			return true;
 	const char *e = getenv("XM_HOST_STAGING");
 	if (e && *e == '0')
 		return false;
@@ -1005,6 +1013,68 @@ stage_budget(double total)
 	HIP_OK("xm_contract", hipMemGetInfo(&fr, &tot));
 	double b = std::max(total / 6.0, (double)(256ull << 20));
 	return (size_t)std::min(b, (double)fr / 4.0);
+}
+
+static size_t
+round256(size_t n)
+{
+	return (n + 255) / 256 * 256;
+}
+
+/*
+ * Lay out a staged batch's file ranges in the pinned buffer at `base`:
+ * its file operand ranges (`stage` entries, which then copy from the
+ * pinned copies) and its file C blocks (the C-in and C-out permutes then
+ * read and write the pinned copies).  base == nullptr: count only.
+ * Returns the buffer bytes the batch needs.
+ */
+static size_t
+file_layout(contract_plan *sub, char *base)
+{
+	const size_t esz = elem_size(sub->type);
+	size_t off = 0;
+	if (base) {
+		sub->file_ops.clear();
+		sub->file_c.clear();
+		sub->file_bytes = 0;
+	}
+	for (auto &cp : sub->stage) {
+		if (!file_ptr(cp.host))
+			continue;
+		if (base) {
+			sub->file_ops.push_back({cp.host, cp.bytes, off});
+			sub->file_bytes += (double)cp.bytes;
+			cp.host = (uint64_t)(uintptr_t)(base + off);
+		}
+		off += round256(cp.bytes);
+	}
+	std::map<uint64_t, size_t> cofs;
+	for (auto &j : sub->cout_jobs) {
+		if (!file_ptr(j.dst))
+			continue;
+		auto it = cofs.find(j.dst);
+		if (it == cofs.end()) {
+			it = cofs.emplace(j.dst, off).first;
+			if (base) {
+				sub->file_c.push_back({j.dst, j.n * esz, off});
+				sub->file_bytes += 2.0 * (double)(j.n * esz);
+			}
+			off += round256(j.n * esz);
+		}
+		if (base)
+			j.dst = (uint64_t)(uintptr_t)(base + it->second);
+	}
+	if (base)
+		for (auto &j : sub->cin_jobs) {
+			if (!file_ptr(j.src))
+				continue;
+			auto it = cofs.find(j.src);
+			if (it == cofs.end())
+				XM_FATAL("xm_contract", "file C block without "
+				    "a write-back");
+			j.src = (uint64_t)(uintptr_t)(base + it->second);
+		}
+	return off;
 }
 
 /*
@@ -1095,9 +1165,21 @@ build_staged(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 	for (int r = 0; r < 2; r++)
 		if (rb[r])
 			HIP_OK(fn, hipMalloc(&pl->region[r], rb[r]));
+	/* file ranges: in the pinned buffer of the batch's region */
+	size_t fb[2] = {0, 0};
+	for (size_t i = 0; i < pl->batches.size(); i++)
+		fb[i % 2] = std::max(fb[i % 2], file_layout(pl->batches[i],
+		    nullptr));
+	for (int r = 0; r < 2; r++)
+		if (fb[r])
+			HIP_OK(fn, hipHostMalloc(&pl->bounce[r], fb[r],
+			    hipHostMallocMapped));
 	for (size_t i = 0; i < pl->batches.size(); i++) {
-		pl->batches[i]->ws = pl->region[i % 2];
-		plan_upload(pl->batches[i]);
+		contract_plan *sub = pl->batches[i];
+		if (fb[i % 2])
+			file_layout(sub, (char *)pl->bounce[i % 2]);
+		sub->ws = pl->region[i % 2];
+		plan_upload(sub);
 	}
 	HIP_OK(fn, hipEventCreateWithFlags(&pl->ev_start, hipEventDisableTiming));
 	for (int r = 0; r < 2; r++) {
@@ -1267,6 +1349,9 @@ run_plan(contract_plan *pl, xm_scalar_t alpha, xm_scalar_t beta)
  * A staged plan's call: batch i computes on the main stream once its host
  * operand blocks are in region i % 2; the side stream copies batch i + 1's
  * blocks meanwhile, after batch i - 1 (the region's previous user) is done.
+ * File ranges: the host reads batch i + 1's into bounce[(i + 1) % 2] while
+ * batch i computes, once batch i - 1 (that buffer's previous user) has
+ * finished and its C blocks are written back to the file.
  */
 static void
 run_staged(contract_plan *pl, xm_scalar_t alpha, xm_scalar_t beta)
@@ -1281,8 +1366,9 @@ run_staged(contract_plan *pl, xm_scalar_t alpha, xm_scalar_t beta)
 	sc.beta_im = beta.imag();
 	sc.flags = alpha == xm_scalar_t(0) ? 1 : 0;
 	sc.flags |= beta == xm_scalar_t(0) ? 2 : 0;
-	const bool pairs = !(sc.flags & 1);
+	const bool pairs = !(sc.flags & 1), cin = !(sc.flags & 2);
 	const size_t nb = pl->batches.size();
+	const bool file = pl->bounce[0] || pl->bounce[1];
 	for (contract_plan *sub : pl->batches) {
 		*sub->host_scalars = sc;
 		HIP_OK(fn, hipMemcpyAsync(sub->d_scalars, sub->host_scalars,
@@ -1291,33 +1377,55 @@ run_staged(contract_plan *pl, xm_scalar_t alpha, xm_scalar_t beta)
 	}
 	auto stage = [&](size_t i) {
 		contract_plan *sub = pl->batches[i];
+		char *bb = (char *)pl->bounce[i % 2];
+		if (pairs)
+			for (auto &f : sub->file_ops)
+				arena_file_read(fn, f.fptr, bb + f.off, f.bytes);
+		if (cin)
+			for (auto &f : sub->file_c)
+				arena_file_read(fn, f.fptr, bb + f.off, f.bytes);
 		char *base = (char *)sub->ws;
-		for (auto &cp : sub->stage)
-			HIP_OK(fn, hipMemcpyAsync(base + cp.off,
-			    (const void *)(uintptr_t)cp.host, cp.bytes,
-			    hipMemcpyHostToDevice, side));
+		if (pairs)
+			for (auto &cp : sub->stage)
+				HIP_OK(fn, hipMemcpyAsync(base + cp.off,
+				    (const void *)(uintptr_t)cp.host, cp.bytes,
+				    hipMemcpyHostToDevice, side));
 		HIP_OK(fn, hipEventRecord(pl->ev_staged[i % 2], side));
 	};
-	if (pairs) {
-		/* the regions may still be read by the previous call */
-		HIP_OK(fn, hipEventRecord(pl->ev_start, s));
-		HIP_OK(fn, hipStreamWaitEvent(side, pl->ev_start, 0));
-		stage(0);
-	}
+	/* batch i's C blocks back to the file (its work has finished) */
+	auto write_back = [&](size_t i) {
+		char *bb = (char *)pl->bounce[i % 2];
+		for (auto &f : pl->batches[i]->file_c)
+			arena_file_write(fn, f.fptr, bb + f.off, f.bytes);
+	};
+	/* the regions may still be read by the previous call */
+	HIP_OK(fn, hipEventRecord(pl->ev_start, s));
+	HIP_OK(fn, hipStreamWaitEvent(side, pl->ev_start, 0));
+	stage(0);
 	for (size_t i = 0; i < nb; i++) {
-		if (pairs && i + 1 < nb) {
-			if (i >= 1)
-				HIP_OK(fn, hipStreamWaitEvent(side,
-				    pl->ev_done[(i + 1) % 2], 0));
-			stage(i + 1);
-		}
-		if (pairs)
-			HIP_OK(fn, hipStreamWaitEvent(s, pl->ev_staged[i % 2], 0));
+		HIP_OK(fn, hipStreamWaitEvent(s, pl->ev_staged[i % 2], 0));
 		launch_sequence(pl->batches[i], sc.flags, s, false);
 		HIP_OK(fn, hipEventRecord(pl->ev_done[i % 2], s));
+		if (i + 1 < nb) {
+			if (i >= 1) {
+				if (file) {
+					HIP_OK(fn, hipEventSynchronize(
+					    pl->ev_done[(i + 1) % 2]));
+					write_back(i - 1);
+				}
+				HIP_OK(fn, hipStreamWaitEvent(side,
+				    pl->ev_done[(i + 1) % 2], 0));
+			}
+			stage(i + 1);
+		}
 	}
 	HIP_OK(fn, hipStreamSynchronize(s));
 	HIP_OK(fn, hipStreamSynchronize(side));
+	if (file) {
+		if (nb >= 2)
+			write_back(nb - 2);
+		write_back(nb - 1);
+	}
 }
 
 /* The plan's device work for one call: permutes in, slices, grouped GEMM,
@@ -1386,7 +1494,7 @@ xm_contract(xm_scalar_t alpha, const xm_tensor_t *a, const xm_tensor_t *b,
  * S's index order and merged with S's own (image_terms): the adjoint's
  * gradient in one grouped GEMM, without per-image outputs or folds.
  * Returns false (nothing done) when an image mixes the dims dx takes
- * from p with those from q, or a block is in host memory.
+ * from p with those from q, or a block is in host memory or in a file.
  */
 bool
 contract_gradient(xm_scalar_t alpha, const xm_tensor_t *p,
@@ -1450,6 +1558,10 @@ record_stats(contract_plan *pl, bool cached, xm_scalar_t alpha,
 		    (cin ? q->cin_l.ngather : 0) + q->cout_l.ngather;
 		if (ops)
 			st.staged_bytes += q->staged_bytes;
+		for (auto &f : q->file_ops)
+			st.file_bytes += ops ? (double)f.bytes : 0;
+		for (auto &f : q->file_c)
+			st.file_bytes += (cin ? 2.0 : 1.0) * (double)f.bytes;
 		gms.push_back((float)ms[1]);
 	}
 	st.total_ms = total_ms;
@@ -1649,8 +1761,8 @@ xm_contract_noalloc(xm_scalar_t alpha, const xm_tensor_t *a,
 	masks_t mk;
 	contract_masks(fn, a, b, c, idxa, idxb, idxc, mk);
 	if (touches_host(a, b, c)) {
-		fprintf(stderr, "libxm: %s: blocks in host memory need staging "
-		    "buffers; use xm_contract\n", fn);
+		fprintf(stderr, "libxm: %s: blocks in host memory or in a file "
+		    "need staging buffers; use xm_contract\n", fn);
 		return -1;
 	}
 	int dev = 0;

@@ -104,6 +104,22 @@ struct contract_plan {
 	void *region[2] = {nullptr, nullptr};
 	hipEvent_t ev_start = nullptr, ev_staged[2] = {}, ev_done[2] = {};
 	double staged_bytes = 0;
+	/*
+	 * Blocks in an allocator's file: a batch reads its file operand
+	 * ranges (file_ops, `stage` entries then copy from the pinned
+	 * copies) and its file C blocks (file_c, read when beta != 0; the
+	 * C-in and C-out permutes use the pinned copies in place) into its
+	 * region's pinned, device-mapped buffer bounce[i % 2] of the parent,
+	 * and writes file_c back once the batch has run.
+	 */
+	struct file_copy {
+		uint64_t fptr;		/* file pointer */
+		size_t bytes;
+		size_t off;		/* offset in the bounce buffer */
+	};
+	std::vector<file_copy> file_ops, file_c;
+	void *bounce[2] = {nullptr, nullptr};
+	double file_bytes = 0;		/* read + written per call */
 
 	/* device memory */
 	size_t ws_bytes = 0;

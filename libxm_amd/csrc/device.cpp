@@ -2,8 +2,8 @@
  * device.cpp - the device context: one non-blocking HIP stream at the
  * default priority for all library work, a second one (same priority) for
  * copies that stage host-resident blocks into HBM, reusable device scratch
- * buffers (descriptor arrays, operand panels) and pinned staging for
- * uploads.
+ * buffers (descriptor arrays, operand panels), pinned staging for
+ * uploads and pinned, device-mapped buffers for file-resident blocks.
  */
 #include "internal.h"
 
@@ -95,6 +95,27 @@ upload(int slot, const void *host, size_t bytes)
 	HIP_OK("libxm", hipMemcpyAsync(d, s.host, bytes, hipMemcpyHostToDevice,
 	    xm_stream()));
 	return d;
+}
+
+static struct {
+	void *p = nullptr;
+	size_t bytes = 0;
+} pinned[NSLOTS];
+
+void *
+pinned_buffer(int slot, size_t bytes)
+{
+	std::lock_guard<std::mutex> g(scratch_mu);
+	auto &s = pinned[slot];
+	if (s.bytes < std::max<size_t>(bytes, 1)) {
+		if (s.p)
+			HIP_OK("libxm", hipHostFree(s.p));
+		size_t nb = std::max(bytes, s.bytes + s.bytes / 2);
+		nb = (nb + 4095) & ~(size_t)4095;
+		HIP_OK("libxm", hipHostMalloc(&s.p, nb, hipHostMallocMapped));
+		s.bytes = nb;
+	}
+	return s.p;
 }
 
 double

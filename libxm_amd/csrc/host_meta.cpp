@@ -1,8 +1,9 @@
 /*
  * host_meta.cpp - host-side metadata of the libxm API: error convention,
  * index masks, multi-index algebra, block spaces, scalar helpers and the
- * block map of tensors.  Block DATA lives in the HBM arena (arena.cpp);
- * this file only touches it through hipMemcpy in read/write helpers.
+ * block map of tensors.  Block DATA lives in the arena (arena.cpp); this
+ * file only touches it through the allocator's read/write helpers, a
+ * hipMemcpy or a pread.
  */
 #include "internal.h"
 
@@ -1016,11 +1017,16 @@ xm_tensor_get_element(const xm_tensor_t *t, xm_dim_t idx)
 		auto it = ecache.blocks.find(src);
 		if (it == ecache.blocks.end() || it->second.size() < bytes) {
 			std::vector<unsigned char> h(bytes);
-			HIP_OK("xm_tensor_get_element", hipMemcpyAsync(h.data(),
-			    (const void *)(uintptr_t)src, bytes,
-			    hipMemcpyDeviceToHost, xm_stream()));
-			HIP_OK("xm_tensor_get_element",
-			    hipStreamSynchronize(xm_stream()));
+			if (file_ptr(src)) {
+				arena_file_read("xm_tensor_get_element", src,
+				    h.data(), bytes);
+			} else {
+				HIP_OK("xm_tensor_get_element", hipMemcpyAsync(
+				    h.data(), (const void *)(uintptr_t)src,
+				    bytes, hipMemcpyDeviceToHost, xm_stream()));
+				HIP_OK("xm_tensor_get_element",
+				    hipStreamSynchronize(xm_stream()));
+			}
 			ecache.bytes += bytes;
 			it = ecache.blocks.insert_or_assign(src, std::move(h)).first;
 		}

@@ -20,6 +20,7 @@
 
 #include "xm.h"
 #include "util.h"
+#include "filestore.h"
 
 #define XM_API extern "C" __attribute__((visibility("default")))
 
@@ -50,8 +51,9 @@ struct xm_block {
 	xm_block_type_t type;
 	xm_dim_t perm;
 	xm_scalar_t scalar;
-	uint64_t data;	/* canonical: device address; derivative: flat index
-			   of the source block in the block grid */
+	uint64_t data;	/* canonical: device address, or a file pointer
+			   (filestore.h); derivative: flat index of the
+			   source block in the block grid */
 };
 
 struct xm_tensor {
@@ -79,6 +81,12 @@ struct xm_allocator {
 	std::vector<arena_slab> slabs;
 	std::vector<std::pair<uint64_t, size_t>> live;	/* addr -> bytes */
 	size_t device_bytes, host_bytes, limit_bytes;
+	size_t host_limit;	/* pinned-host tier cap */
+	/* file tier (allocators with a path): the backing file, its slot
+	 * in file pointers, bytes of live file ranges */
+	file_store *file;
+	uint32_t file_slot;
+	size_t file_bytes;
 	uint64_t id;
 };
 
@@ -115,6 +123,50 @@ arena_is_host(xm_allocator_t *a, uint64_t ptr)
 {
 	return arena_host_slab(a, ptr) != 0;
 }
+/* pread / pwrite of bytes at a file pointer (fatal on I/O errors). */
+void arena_file_read(const char *fn, uint64_t ptr, void *buf, size_t bytes);
+void arena_file_write(const char *fn, uint64_t ptr, const void *buf,
+    size_t bytes);
+/* Does any canonical block of t live in a file? */
+bool tensor_in_file(const xm_tensor_t *t);
+/* Pinned, device-mapped host scratch of at least `bytes`, grown on demand
+ * (slots as for scratch_buffer; the caller has synchronized the work
+ * that used the slot before). */
+void *pinned_buffer(int slot, size_t bytes);
+
+/*
+ * Pinned copies of the file ranges one batch of kernel jobs reads and
+ * writes: ranges only read (rd) and ranges written (wr) lie in separate
+ * parts of one pinned, device-mapped buffer, so a job reading a block
+ * that another job of the batch writes sees the values from before the
+ * batch.  load() reads rd and the loaded wr ranges from the files; the
+ * kernels use src() / dst() addresses; store() writes wr back.
+ */
+struct file_stage {
+	stage_layout rd, wr;
+	char *buf = nullptr;
+	int slot;
+
+	explicit file_stage(int pinned_slot) : slot(pinned_slot) {}
+	void need_read(uint64_t p, uint64_t bytes)
+	{
+		if (file_ptr(p))
+			rd.add(p, bytes, true, false);
+	}
+	void need_write(uint64_t p, uint64_t bytes, bool load)
+	{
+		if (file_ptr(p))
+			wr.add(p, bytes, load, true);
+	}
+	bool empty() const { return rd.empty() && wr.empty(); }
+	void load(const char *fn);
+	uint64_t src(uint64_t p) const;
+	uint64_t dst(uint64_t p) const;
+	void store(const char *fn);
+};
+/* Bytes a batch of file-staged element jobs aims at: XM_STAGE_BYTES, else
+ * 1 GiB. */
+size_t file_batch_bytes(void);
 /* Reusable device scratch of at least `bytes`, grown on demand; slot
  * selects independent buffers.  Valid until the next call with the same
  * slot. */

@@ -1,7 +1,10 @@
 /*
  * ops.cpp - tensor-wide block operations on the GPU
  * (libxm src/xm.c:28-508).  Each call builds one job per canonical block
- * of the output and runs one kernel over all of them.
+ * of the output and runs one kernel over all of them.  Blocks in an
+ * allocator's file are read into pinned, device-mapped buffers first and
+ * the written ones written back (file_stage), in batches of about
+ * XM_STAGE_BYTES (1 GiB by default).
  */
 #include "contract_plan.h"
 
@@ -124,7 +127,7 @@ stage_aliased(const xm_tensor_t *a, std::vector<elem_job> &jobs,
 		dsts.insert(j.dst);
 	std::map<uint64_t, size_t> need;	/* src ptr -> elements */
 	for (auto &j : jobs)
-		if (!j.src_zero && dsts.count(j.src))
+		if (!j.src_zero && !file_ptr(j.src) && dsts.count(j.src))
 			need[j.src] = std::max<size_t>(need[j.src], j.n);
 	if (need.empty())
 		return;
@@ -246,6 +249,91 @@ xm_amd_last_permute(size_t *tiled, size_t *gather)
 		*gather = last_launch.ngather;
 }
 
+static uint64_t
+stage_round(uint64_t n)
+{
+	return (n + 255) / 256 * 256;
+}
+
+static bool
+jobs_touch_file(const std::vector<elem_job> &jobs)
+{
+	for (const auto &j : jobs)
+		if (file_ptr(j.dst) || (!j.src_zero && file_ptr(j.src)))
+			return true;
+	return false;
+}
+
+/*
+ * Element jobs of which some read or write blocks in a file: batches of
+ * consecutive jobs whose staged ranges stay within file_batch_bytes()
+ * (a job larger than that runs alone), each batch's file blocks read
+ * into pinned memory, the kernel run on the pinned copies, the written
+ * blocks written back.  A list in which some job reads a block that a
+ * job writes runs as one batch, so that every job reads the values from
+ * before the call.
+ */
+static elem_launch
+run_file_jobs(const char *fn, const std::vector<elem_job> &jobs, int op,
+    xm_scalar_type_t dt, xm_scalar_type_t st, xm_scalar_t alpha,
+    const xm_tensor_t *a)
+{
+	hipStream_t s = xm_stream();
+	const size_t desz = elem_size(dt), sesz = elem_size(st);
+	std::set<uint64_t> dsts;
+	for (auto &j : jobs)
+		dsts.insert(j.dst);
+	bool aliased = false;
+	for (auto &j : jobs)
+		aliased |= !j.src_zero && dsts.count(j.src) != 0;
+	const uint64_t budget = aliased ? UINT64_MAX : file_batch_bytes();
+	/* ELEM_COPY_SCALED writes whole blocks without reading them */
+	const bool load_dst = op != ELEM_COPY_SCALED;
+	elem_launch total;
+	size_t i = 0;
+	while (i < jobs.size()) {
+		file_stage fs(0);
+		std::set<std::pair<uint64_t, bool>> seen;	/* ptr, written */
+		uint64_t pend = 0;
+		size_t e = i;
+		for (; e < jobs.size(); e++) {
+			const elem_job &j = jobs[e];
+			const bool wd = file_ptr(j.dst) && !seen.count({j.dst, true});
+			const bool rs = !j.src_zero && file_ptr(j.src) &&
+			    !seen.count({j.src, false});
+			const uint64_t cost = (wd ? stage_round(j.n * desz) : 0) +
+			    (rs ? stage_round(j.n * sesz) : 0);
+			if (e > i && pend + cost > budget)
+				break;
+			pend += cost;
+			if (wd) {
+				fs.need_write(j.dst, j.n * desz, load_dst);
+				seen.insert({j.dst, true});
+			}
+			if (rs) {
+				fs.need_read(j.src, j.n * sesz);
+				seen.insert({j.src, false});
+			}
+		}
+		fs.load(fn);
+		std::vector<elem_job> b(jobs.begin() + i, jobs.begin() + e);
+		for (auto &j : b) {
+			j.dst = fs.dst(j.dst);
+			if (!j.src_zero)
+				j.src = fs.src(j.src);
+		}
+		stage_aliased(a, b, sesz, s);
+		elem_launch l = launch_elem(b.data(), b.size(), op, dt, st,
+		    alpha, s);
+		HIP_OK(fn, hipStreamSynchronize(s));
+		fs.store(fn);
+		total.ngather += l.ngather;
+		total.ntiled += l.ntiled;
+		i = e;
+	}
+	return total;
+}
+
 /* Run the jobs `build` makes (or their cached device copy) as `op`. */
 template <typename B>
 static void
@@ -257,6 +345,11 @@ run_jobs(const char *fn, const op_key &key, B build, int op,
 	op_entry e;
 	if (!op_cache_find(key, &e)) {
 		std::vector<elem_job> jobs = build();
+		if (jobs_touch_file(jobs)) {
+			last_launch = run_file_jobs(fn, jobs, op, dt, st, alpha,
+			    a);
+			return;
+		}
 		std::set<uint64_t> dsts;
 		bool aliased = false;
 		for (auto &j : jobs)
@@ -280,25 +373,87 @@ run_jobs(const char *fn, const op_key &key, B build, int op,
 	HIP_OK(fn, hipStreamSynchronize(s));
 }
 
+/* One segment per canonical block of t with elements. */
+static std::vector<seg_desc>
+canonical_segs(const xm_tensor_t *t, size_t *nchunks)
+{
+	std::vector<seg_desc> segs;
+	size_t c = 0;
+	for (size_t f = 0; f < t->blocks.size(); f++) {
+		if (t->blocks[f].type != XM_BLOCK_TYPE_CANONICAL)
+			continue;
+		xm_dim_t d = block_dims_flat(t, f);
+		seg_desc sd{t->blocks[f].data, xm_dim_dot(&d), c, f};
+		if (sd.n == 0)
+			continue;
+		c += (sd.n + ELEM_CHUNK - 1) / ELEM_CHUNK;
+		segs.push_back(sd);
+	}
+	*nchunks = c;
+	return segs;
+}
+
+/*
+ * A fill kernel over segments of which some lie in a file: batches of
+ * segments within file_batch_bytes(), each filled in pinned copies that
+ * are then written back.  launch(dsegs, nseg, nchunks, stream) runs the
+ * kernel on a device list.
+ */
+template <typename L>
+static void
+run_file_segs(const char *fn, const std::vector<seg_desc> &segs, size_t esz,
+    L launch)
+{
+	hipStream_t s = xm_stream();
+	const uint64_t budget = file_batch_bytes();
+	size_t i = 0;
+	while (i < segs.size()) {
+		file_stage fs(0);
+		uint64_t pend = 0;
+		size_t e = i;
+		for (; e < segs.size(); e++) {
+			const uint64_t cost = file_ptr(segs[e].ptr) ?
+			    stage_round(segs[e].n * esz) : 0;
+			if (e > i && pend + cost > budget)
+				break;
+			pend += cost;
+			fs.need_write(segs[e].ptr, segs[e].n * esz, false);
+		}
+		fs.load(fn);
+		std::vector<seg_desc> b(segs.begin() + i, segs.begin() + e);
+		size_t c = 0;
+		for (auto &sd : b) {
+			sd.ptr = fs.dst(sd.ptr);
+			sd.first_chunk = c;
+			c += (sd.n + ELEM_CHUNK - 1) / ELEM_CHUNK;
+		}
+		const seg_desc *d = (const seg_desc *)upload(1, b.data(),
+		    b.size() * sizeof(seg_desc));
+		launch(d, b.size(), c, s);
+		HIP_OK(fn, hipStreamSynchronize(s));
+		fs.store(fn);
+		i = e;
+	}
+}
+
 /* libxm src/xm.c:28-64 */
 XM_API void
 xm_set(xm_tensor_t *a, xm_scalar_t x)
 {
+	if (tensor_in_file(a)) {
+		size_t c;
+		run_file_segs("xm_set", canonical_segs(a, &c),
+		    elem_size(a->type), [&](const seg_desc *d, size_t n,
+		    size_t nc, hipStream_t s) {
+			launch_fill_value(d, n, nc, a->type, x, s);
+		});
+		return;
+	}
 	const op_key key = make_key(-1, a, nullptr, nullptr, nullptr, 0);
 	op_entry e;
 	if (!op_cache_find(key, &e)) {
-		std::vector<seg_desc> segs;
-		size_t c = 0;
-		for (size_t f = 0; f < a->blocks.size(); f++) {
-			if (a->blocks[f].type != XM_BLOCK_TYPE_CANONICAL)
-				continue;
-			xm_dim_t d = block_dims_flat(a, f);
-			seg_desc sd{a->blocks[f].data, xm_dim_dot(&d), c, f};
-			if (sd.n == 0)
-				continue;
-			c += (sd.n + ELEM_CHUNK - 1) / ELEM_CHUNK;
-			segs.push_back(sd);
-		}
+		size_t c;
+		std::vector<seg_desc> segs = canonical_segs(a, &c);
 		elem_launch l;
 		l.ngather = segs.size();
 		l.gchunks = c;
@@ -317,24 +472,23 @@ XM_API void
 xm_amd_fill_random(xm_tensor_t *t, uint64_t seed, double lo, double hi,
     double wlo, double whi)
 {
-	std::vector<seg_desc> segs;
-	size_t c = 0;
-	for (size_t f = 0; f < t->blocks.size(); f++) {
-		if (t->blocks[f].type != XM_BLOCK_TYPE_CANONICAL)
-			continue;
-		xm_dim_t d = block_dims_flat(t, f);
-		seg_desc sd{t->blocks[f].data, xm_dim_dot(&d), c, f};
-		if (sd.n == 0)
-			continue;
-		c += (sd.n + ELEM_CHUNK - 1) / ELEM_CHUNK;
-		segs.push_back(sd);
-	}
+	size_t c;
+	std::vector<seg_desc> segs = canonical_segs(t, &c);
 	if (segs.empty())
 		return;
+	auto launch = [&](const seg_desc *d, size_t n, size_t nc,
+	    hipStream_t s) {
+		launch_fill_random(d, n, nc, t->type, seed, lo, hi, wlo, whi,
+		    s);
+	};
+	if (tensor_in_file(t)) {
+		run_file_segs("xm_amd_fill_random", segs, elem_size(t->type),
+		    launch);
+		return;
+	}
 	hipStream_t s = xm_stream();
-	void *d = upload(1, segs.data(), segs.size() * sizeof(seg_desc));
-	launch_fill_random((const seg_desc *)d, segs.size(), c, t->type, seed,
-	    lo, hi, wlo, whi, s);
+	launch((const seg_desc *)upload(1, segs.data(),
+	    segs.size() * sizeof(seg_desc)), segs.size(), c, s);
 	HIP_OK("xm_amd_fill_random", hipStreamSynchronize(s));
 }
 
@@ -425,17 +579,19 @@ xm_div(xm_tensor_t *a, const xm_tensor_t *b, const char *idxa,
 /* libxm src/xm.c:415-508: sum over all nonzero block pairs of
  * s_a*s_b*dot(unfold(a), unfold(b)).  Derivative a blocks are first laid
  * out in logical order in scratch; per-chunk partials are summed on the
- * host in a fixed order. */
+ * host in a fixed order (the order of the jobs, also when blocks in a
+ * file make the jobs run in several batches). */
 XM_API xm_scalar_t
 xm_dot(const xm_tensor_t *a, const xm_tensor_t *b, const char *idxa,
     const char *idxb)
 {
+	const char *fn = "xm_dot";
 	xm_dim_t ca, cb;
-	pair_masks("xm_dot", a, b, idxa, idxb, true, &ca, &cb);
+	pair_masks(fn, a, b, idxa, idxb, true, &ca, &cb);
 	hipStream_t s = xm_stream();
 	size_t esz = elem_size(a->type);
 	std::vector<elem_job> gather, dots;
-	size_t ws = 0;
+	std::vector<int> gi;		/* gather job of each dot job, or -1 */
 	xm_dim_t ib = xm_dim_zero(cb.n);
 	for (size_t fa = 0; fa < a->blocks.size(); fa++) {
 		const xm_block &ba = a->blocks[fa];
@@ -464,15 +620,13 @@ xm_dot(const xm_tensor_t *a, const xm_tensor_t *b, const char *idxa,
 		    b->blocks[bb.data].data : bb.data;
 		uint64_t asrc = ba.type == XM_BLOCK_TYPE_DERIVATIVE ?
 		    a->blocks[ba.data].data : ba.data;
+		gi.push_back(-1);
 		if (ba.type == XM_BLOCK_TYPE_DERIVATIVE) {
 			elem_job g = j;
 			g.src = asrc;
 			block_strides(a, fa, g.sstride);
-			g.dst = ws;		/* relocated below */
-			ws += (j.n * esz + 255) / 256 * 256;
-			gather.push_back(g);
-			j.dst = g.dst;
-			j.src_zero = 1;		/* marks: dst is in scratch */
+			gi.back() = (int)gather.size();
+			gather.push_back(g);	/* dst: scratch, set per batch */
 		} else {
 			j.dst = asrc;
 		}
@@ -483,28 +637,78 @@ xm_dot(const xm_tensor_t *a, const xm_tensor_t *b, const char *idxa,
 	}
 	if (dots.empty())
 		return 0;
-	char *buf = (char *)scratch_buffer(3, ws ? ws : 256);
-	for (auto &g : gather)
-		g.dst += (uint64_t)(uintptr_t)buf;
-	for (auto &j : dots) {
-		if (j.src_zero)
-			j.dst += (uint64_t)(uintptr_t)buf;
-		j.src_zero = 0;
+	std::vector<double> hp;		/* partials of all batches */
+	/* dot jobs [i0, i1), file blocks read through fs */
+	auto run = [&](size_t i0, size_t i1, const file_stage &fs) {
+		std::vector<elem_job> g, d(dots.begin() + i0,
+		    dots.begin() + i1);
+		size_t ws = 0;
+		for (size_t k = i0; k < i1; k++)
+			if (gi[k] >= 0)
+				ws += stage_round(dots[k].n * esz);
+		char *buf = (char *)scratch_buffer(3, ws ? ws : 256);
+		ws = 0;
+		for (size_t k = i0; k < i1; k++) {
+			elem_job &j = d[k - i0];
+			j.src = fs.src(j.src);
+			if (gi[k] >= 0) {
+				elem_job x = gather[gi[k]];
+				x.src = fs.src(x.src);
+				x.dst = (uint64_t)(uintptr_t)(buf + ws);
+				ws += stage_round(j.n * esz);
+				j.dst = x.dst;
+				g.push_back(x);
+			} else {
+				j.dst = fs.src(j.dst);
+			}
+		}
+		if (!g.empty()) {
+			launch_elem(g.data(), g.size(), ELEM_GATHER, a->type,
+			    a->type, 0, s);
+			HIP_OK(fn, hipStreamSynchronize(s));
+		}
+		size_t nc = set_chunks(d);
+		double *dp = (double *)scratch_buffer(4, 2 * nc * sizeof(double));
+		launch_dot(d.data(), d.size(), nc, a->type, dp, s);
+		const size_t h0 = hp.size();
+		hp.resize(h0 + 2 * nc);
+		HIP_OK(fn, hipMemcpyAsync(hp.data() + h0, dp,
+		    2 * nc * sizeof(double), hipMemcpyDeviceToHost, s));
+		HIP_OK(fn, hipStreamSynchronize(s));
+	};
+	if (!tensor_in_file(a) && !tensor_in_file(b)) {
+		run(0, dots.size(), file_stage(0));
+	} else {
+		const uint64_t budget = file_batch_bytes();
+		size_t i = 0;
+		while (i < dots.size()) {
+			file_stage fs(0);
+			std::set<uint64_t> seen;
+			uint64_t pend = 0;
+			size_t e = i;
+			for (; e < dots.size(); e++) {
+				const uint64_t ra = gi[e] >= 0 ?
+				    gather[gi[e]].src : dots[e].dst;
+				const uint64_t bytes = dots[e].n * esz;
+				uint64_t cost = 0;
+				for (uint64_t p : {ra, dots[e].src})
+					if (file_ptr(p) && !seen.count(p))
+						cost += stage_round(bytes);
+				if (e > i && pend + cost > budget)
+					break;
+				pend += cost;
+				for (uint64_t p : {ra, dots[e].src}) {
+					fs.need_read(p, bytes);
+					seen.insert(p);
+				}
+			}
+			fs.load(fn);
+			run(i, e, fs);
+			i = e;
+		}
 	}
-	if (!gather.empty()) {
-		launch_elem(gather.data(), gather.size(), ELEM_GATHER,
-		    a->type, a->type, 0, s);
-		HIP_OK("xm_dot", hipStreamSynchronize(s));
-	}
-	size_t nc = set_chunks(dots);
-	double *dp = (double *)scratch_buffer(4, 2 * nc * sizeof(double));
-	launch_dot(dots.data(), dots.size(), nc, a->type, dp, s);
-	std::vector<double> hp(2 * nc);
-	HIP_OK("xm_dot", hipMemcpyAsync(hp.data(), dp, hp.size() * sizeof(double),
-	    hipMemcpyDeviceToHost, s));
-	HIP_OK("xm_dot", hipStreamSynchronize(s));
 	zdouble sum = 0;
-	for (size_t i = 0; i < nc; i++)
+	for (size_t i = 0; i < hp.size() / 2; i++)
 		sum += zdouble(hp[2 * i], hp[2 * i + 1]);
 	if (a->type == XM_SCALAR_FLOAT || a->type == XM_SCALAR_DOUBLE)
 		sum = sum.real();

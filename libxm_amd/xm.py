@@ -70,7 +70,7 @@ class ContractStats(C.Structure):
                 ("permute_gather", C.c_size_t),
                 ("staged_batches", C.c_int), ("staged_bytes", C.c_double),
                 ("ksplit_tiles", C.c_size_t), ("err_bound", C.c_double),
-                ("gemm_kernel", C.c_char * 32)]
+                ("gemm_kernel", C.c_char * 32), ("file_bytes", C.c_double)]
 
 
 def dim(seq):
@@ -195,6 +195,7 @@ _SIGNATURES = {
     "xm_contract_noalloc": (C.c_int, [Scalar, _P, _P, Scalar, _P, _S, _S, _S]),
     "xm_amd_fill_random": (None, [_P, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double]),
     "xm_amd_allocator_usage": (None, [_P, C.POINTER(_SZ), C.POINTER(_SZ)]),
+    "xm_amd_allocator_file_bytes": (_SZ, [_P]),
     "xm_amd_synchronize": (C.c_int, []),
     "xm_amd_last_permute": (None, [C.POINTER(_SZ), C.POINTER(_SZ)]),
     "xm_oracle_blas_name": (_S, []),
@@ -419,6 +420,9 @@ class XmLib:
         d, h = C.c_size_t(), C.c_size_t()
         self.lib.xm_amd_allocator_usage(alloc, C.byref(d), C.byref(h))
         return d.value, h.value
+
+    def allocator_file_bytes(self, alloc):
+        return int(self.lib.xm_amd_allocator_file_bytes(alloc))
 
     def last_permute(self):
         """(tiled, gather) jobs of the last add/copy/mul/div."""
