@@ -1,13 +1,15 @@
 /*
  * alloc.h - block storage of the libxm C API (MI355X build).
  *
- * Same prototypes as libxm's src/alloc.h.  Storage is an arena in the
- * GPU's HBM: data_ptr is an opaque 64-bit handle (here the device address
- * of the allocation).  xm_allocator_read/write copy between a host buffer
- * and the device allocation and return when the copy has completed.
- * When HBM is exhausted (or the arena limit set by the environment
- * variable XM_ARENA_BYTES is reached) allocations fall back to pinned,
- * device-mapped host memory so that tensors larger than HBM still work.
+ * Same prototypes as libxm's src/alloc.h.  data_ptr is an opaque 64-bit
+ * handle.  Allocations come from an arena in the GPU's HBM (data_ptr is
+ * then the device address); once HBM is exhausted, or the arena reaches
+ * the limit set by the environment variable XM_ARENA_BYTES, from pinned,
+ * device-mapped host memory (up to XM_HOST_BYTES); and for an allocator
+ * created with a path, once that tier is full too, from the file at that
+ * path, as libxm's pagefile allocator.  xm_allocator_read/write copy
+ * between a host buffer and the allocation and return when the copy has
+ * completed.
  */
 #ifndef XM_ALLOC_H
 #define XM_ALLOC_H
@@ -25,9 +27,10 @@ extern "C" {
 /* Opaque allocator (libxm src/alloc.h:34). */
 typedef struct xm_allocator xm_allocator_t;
 
-/* Create an allocator.  path == NULL or not, data live in HBM; the path is
- * only remembered and returned by xm_allocator_get_path
- * (libxm src/alloc.h:41). */
+/* Create an allocator (libxm src/alloc.h:41).  With a path, the file is
+ * created (open with O_CREAT, truncated; NULL if that fails, as libxm)
+ * and holds the allocations that do not fit HBM and pinned host memory;
+ * xm_allocator_destroy removes it. */
 xm_allocator_t *xm_allocator_create(const char *path);
 /* libxm src/alloc.h:46 */
 const char *xm_allocator_get_path(xm_allocator_t *allocator);
