@@ -447,6 +447,13 @@ def precision_mode(lib, args):
     print(json.dumps(out), flush=True)
 
 
+# roofline.traffic is not measured in the timed run: it is the HBM bytes per
+# launch from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this workload,
+# scaled by the same counters' readings for a copy of known size
+TRAFFIC_SOURCE = ("profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE / "
+                  "WRITE_SIZE of this workload, copy-calibrated (read x2)")
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed PMC summary of
     this workload (profiles/traffic.json), or None."""
@@ -697,6 +704,7 @@ def main():
                      # the tiles split in K are summed by k_ksplit_sum
                      "traffic": stage_traffic(("k_gemm_f64_dma",
                                                "k_ksplit_sum")),
+                     "traffic_source": TRAFFIC_SOURCE,
                      "kernel": "k_gemm_f64_dma", "kernel_ms": gemm_ms,
                      "ksplit_tiles": st["ksplit_tiles"],
                      "tile": [st["tile_p"], st["tile_q"]],
@@ -723,6 +731,7 @@ def main():
                          "peak": BF16_PEAK, "unit": "TFLOP/s (BF16 MFMA)",
                          "frac": F * nprod / (sgemm * 1e-3) / 1e12 / BF16_PEAK,
                          "traffic": measured_traffic("k_gemm_split2"),
+                         "traffic_source": TRAFFIC_SOURCE,
                          "kernel": "k_gemm_split2", "kernel_ms": sgemm}}
     if not args.no_split:
         lib.set_precision(xm.PRECISION_SPLIT_INT8, None, 0)
@@ -749,6 +758,9 @@ def main():
                          "achieved": F * nmod / (igemm * 1e-3) / 1e12,
                          "peak": INT8_PEAK, "unit": "TOP/s (int8 MFMA)",
                          "frac": F * nmod / (igemm * 1e-3) / 1e12 / INT8_PEAK,
+                         "traffic": stage_traffic(("k_gemm_crt",
+                                                   "k_crt_recon")),
+                         "traffic_source": TRAFFIC_SOURCE,
                          "kernel": "k_gemm_crt", "kernel_ms": igemm}}
     if not args.no_cpu:
         try:

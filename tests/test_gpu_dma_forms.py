@@ -5,7 +5,9 @@ DMAs (four rows of a column per lane) instead of dword DMAs; FP64 plans
 with even extents run the DMA-staged 64 x 128 kernel instead of the
 register-staged one.  Each pair runs the same MFMA sequence per output, so
 results must be bit-identical (XM_GEMM_DMA=0 selects the alternative),
-and match the oracle to the type's tolerance.
+and match the oracle to the type's tolerance.  The kernel each call ran
+(stats gemm_kernel) is checked too, and a plan with extents that are not
+multiples of 4 must fall back to the dword form.
 """
 import os
 
@@ -30,6 +32,19 @@ def no_dma_env():
         os.environ["XM_GEMM_DMA"] = old
 
 
+# the GEMM kernel per (type, tile, XM_GEMM_DMA)
+KERNEL = {
+    (xm.SCALAR_FLOAT, "64x128", "1"): "k_gemm_mfma_dw<x4>",
+    (xm.SCALAR_FLOAT, "64x128", "0"): "k_gemm_mfma_dw",
+    (xm.SCALAR_FLOAT, "64x64", "1"): "k_gemm_mfma_dw<x4>",
+    (xm.SCALAR_FLOAT, "64x64", "0"): "k_gemm_mfma_dw",
+    (xm.SCALAR_DOUBLE, "64x128", "1"): "k_gemm_f64_dma",
+    (xm.SCALAR_DOUBLE, "64x128", "0"): "k_gemm_f64",
+    (xm.SCALAR_DOUBLE, "64x64", "1"): "k_gemm_f64",
+    (xm.SCALAR_DOUBLE, "64x64", "0"): "k_gemm_f64",
+}
+
+
 @pytest.mark.parametrize("t,tol", [(xm.SCALAR_FLOAT, 1e-5),
                                    (xm.SCALAR_DOUBLE, 1e-13)],
                          ids=["f32", "f64"])
@@ -38,6 +53,18 @@ def test_dma_forms_identical(gpulib, oracle, no_dma_env, t, tol, tile,
                              monkeypatch):
     monkeypatch.setenv("XM_GEMM_TILE", tile)
     w = workloads.ladder_d2h(32, 96, 11, type_=t)
+    run_forms(gpulib, oracle, w, t, tol, lambda dma: KERNEL[t, tile, dma])
+
+
+def test_x4_fallback(gpulib, oracle, no_dma_env):
+    """An FP32 ring with block extents of 4..10 (P or Q not divisible by
+    4): the dword form with DMA on or off, still matching the oracle."""
+    t = xm.SCALAR_FLOAT
+    w = workloads.ring_d2h(20, 48, 3, type_=t, lo=4, hi=10)
+    run_forms(gpulib, oracle, w, t, 1e-5, lambda dma: "k_gemm_mfma_dw")
+
+
+def run_forms(gpulib, oracle, w, t, tol, kernel):
     data = None
     out = {}
     for dma in ("1", "0"):
@@ -49,6 +76,7 @@ def test_dma_forms_identical(gpulib, oracle, no_dma_env, t, tol, tile,
         if data is None:
             data = [s.read(gpulib, z) for s, z in zip(w.specs, x)]
         gpulib.contract(0.5, x[0], x[1], 1.0, x[2], *w.idx)
+        assert gpulib.stats()["gemm_kernel"] == kernel(dma), dma
         out[dma] = w.c.read(gpulib, x[2])
         for z in x:
             gpulib.tensor_free(z)
