@@ -140,16 +140,30 @@ k_crt_norm(const crt_inst *__restrict__ insts, uint32_t ninst)
 	if (threadIdx.x == 0)
 		found = find_first(insts, ninst, wg, &crt_inst::first_norm_wg);
 	__syncthreads();
-	const crt_inst in = insts[found];
+	/* uniform: the descriptor comes in scalar loads */
+	const crt_inst in = insts[__builtin_amdgcn_readfirstlane(found)];
 	const uint32_t r = threadIdx.x % 64, part = threadIdx.x / 64;
 	const uint32_t row = (uint32_t)(wg - in.first_norm_wg) * 64 + r;
-	const double *X = (const double *)(uintptr_t)in.src;
+	typedef const __attribute__((address_space(1))) double gdouble;
 	double s = 0.0;
-	if (row < in.rows)
-		for (uint32_t k = part; k < in.k; k += 4) {
-			const double x = X[(size_t)row + (size_t)k * in.ld];
+	if (row < in.rows) {
+		/* eight loads in flight, the sum in the order of k */
+		const gdouble *Xr = (const gdouble *)(uintptr_t)in.src + row;
+		uint32_t k = part;
+		for (; k + 28 < in.k; k += 32) {
+			double x[8];
+#pragma unroll
+			for (int j = 0; j < 8; j++)
+				x[j] = Xr[(size_t)(k + 4 * j) * in.ld];
+#pragma unroll
+			for (int j = 0; j < 8; j++)
+				s = fma(x[j], x[j], s);
+		}
+		for (; k < in.k; k += 4) {
+			const double x = Xr[(size_t)k * in.ld];
 			s = fma(x, x, s);
 		}
+	}
 	part_sum[part][r] = s;
 	__syncthreads();
 	if (part == 0 && row < in.rows)
@@ -189,6 +203,26 @@ k_crt_scale(const crt_slab *__restrict__ slabs, uint32_t nslab,
 	((int *)(uintptr_t)sl.exps)[row] = e;
 }
 
+/* The 16 residue bytes of a lane for moduli I .. N - 1 (I a template
+ * parameter, so the table entries are immediates). */
+template <int N, int I = 0>
+__device__ __forceinline__ void
+quant_planes(const float (&lm)[16][4], const int (&l0)[16], uint8_t *dst,
+    size_t plane)
+{
+	if constexpr (I < N) {
+		uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+		for (int j = 0; j < 16; j++) {
+			const int r = crt_residue13<I>(lm[j], l0[j], CT);
+			w[j / 4] |= (uint32_t)(r & 0xff) << (8 * (j % 4));
+		}
+		*(uint4 *)(dst + (size_t)I * plane) = make_uint4(w[0], w[1],
+		    w[2], w[3]);
+		quant_planes<N, I + 1>(lm, l0, dst, plane);
+	}
+}
+
 /*
  * One workgroup per (32-row tile, 4 k steps of 32) of an instance.  Thread
  * (L, part) = (tid % 64, tid / 64) owns lane L of the step's fragments:
@@ -209,7 +243,7 @@ k_crt_quant(const crt_inst *__restrict__ insts, uint32_t ninst,
 	if (threadIdx.x == 0)
 		found = find_first(insts, ninst, wg, &crt_inst::first_quant_wg);
 	__syncthreads();
-	const crt_inst in = insts[found];
+	const crt_inst in = insts[__builtin_amdgcn_readfirstlane(found)];
 	const uint32_t local = (uint32_t)(wg - in.first_quant_wg);
 	const uint32_t nkq = (in.nks + 3) / 4;
 	const uint32_t rt = local / nkq, ks = (local % nkq) * 4 + threadIdx.x / 64;
@@ -219,30 +253,33 @@ k_crt_quant(const crt_inst *__restrict__ insts, uint32_t ninst,
 	const uint32_t k0 = ks * 32 + 16 * (L / 32);
 	const bool rin = row < in.rows;
 	const int e = rin ? ((const int *)(uintptr_t)slabs[in.slab].exps)[row] : 0;
-	const double *X = (const double *)(uintptr_t)in.src;
-	int h[16], md[16], lo[16];
+	/* all 16 loads issued before the first is used: entries past the
+	 * instance read its first element instead and are dropped */
+	typedef const __attribute__((address_space(1))) double gdouble;
+	const gdouble *X0 = (const gdouble *)(uintptr_t)in.src;
+	const gdouble *Xr = X0 + row + (size_t)k0 * in.ld;
+	double xv[16];
+#pragma unroll
+	for (int j = 0; j < 16; j++) {
+		const bool ok = rin && k0 + j < in.k;
+		xv[j] = *(ok ? Xr + (size_t)j * in.ld : X0);
+	}
+	float lm[16][4];
+	int l0[16];
 #pragma unroll
 	for (int j = 0; j < 16; j++) {
 		const bool ok = rin && k0 + j < in.k && e != CRT_POISON;
-		const double x = ok ? X[(size_t)row + (size_t)(k0 + j) * in.ld] : 0.0;
-		crt_limbs(rint(ldexp(x, e)), &h[j], &md[j], &lo[j]);
+		int li[4];
+		crt_limbs13(rint(ldexp(ok ? xv[j] : 0.0, e)), li);
+#pragma unroll
+		for (int t = 0; t < 4; t++)
+			lm[j][t] = (float)li[t];
+		l0[j] = li[0];
 	}
 	uint8_t *base = (uint8_t *)(uintptr_t)in.planes;
 	const size_t plane = (size_t)in.nrt * in.nks * 1024;
 	const size_t off = ((size_t)rt * in.nks + ks) * 1024 + 16 * L;
-#pragma unroll
-	for (int l = 0; l < N; l++) {
-		uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-		for (int j = 0; j < 16; j++) {
-			const int r = crt_residue(h[j], md[j], lo[j], CT.p34[l],
-			    CT.p17[l], CRT_MOD[l], CT.finv[l], CT.lo[l],
-			    CT.hi[l]);
-			w[j / 4] |= (uint32_t)(r & 0xff) << (8 * (j % 4));
-		}
-		*(uint4 *)(base + (size_t)l * plane + off) =
-		    make_uint4(w[0], w[1], w[2], w[3]);
-	}
+	quant_planes<N>(lm, l0, base + off, plane);
 }
 
 /* ---- GEMM ------------------------------------------------------------ */
@@ -521,7 +558,7 @@ k_crt_recon(const crt_cblock *__restrict__ ccbs, uint32_t ncb,
 	if (threadIdx.x == 0)
 		found = find_first(ccbs, ncb, wg, &crt_cblock::first_wg);
 	__syncthreads();
-	const crt_cblock cb = ccbs[found];
+	const crt_cblock cb = ccbs[__builtin_amdgcn_readfirstlane(found)];
 	const gemm_scalars sc = *scp;
 	const size_t pq = (size_t)cb.p * cb.q;
 	const size_t e0 = (size_t)(wg - cb.first_wg) * RECON_CHUNK;

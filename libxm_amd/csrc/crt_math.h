@@ -41,6 +41,7 @@ struct crt_tables {
 	float finv[CRT_MAXN];		/* 1 / m, rounded to FP32 */
 	int c31[CRT_MAXN];		/* 2^31 mod m, in [0, m) */
 	int p17[CRT_MAXN], p34[CRT_MAXN];	/* 2^17, 2^34 mod m (symmetric) */
+	int p13[CRT_MAXN], p26[CRT_MAXN], p39[CRT_MAXN]; /* 2^13, 2^26, 2^39 */
 	/* Garner: ginv[i] = (m_0 ... m_{i-1})^-1 mod m_i and
 	 * pm[i][j] = m_0 ... m_{j-1} mod m_i (j < i), symmetric */
 	int ginv[CRT_MAXN];
@@ -91,6 +92,9 @@ crt_make_tables(void)
 		t.c31[i] = (int)(0x80000000ull % m);
 		t.p17[i] = crt_sym(crt_pow2_mod(17, m), m);
 		t.p34[i] = crt_sym(crt_pow2_mod(34, m), m);
+		t.p13[i] = crt_sym(crt_pow2_mod(13, m), m);
+		t.p26[i] = crt_sym(crt_pow2_mod(26, m), m);
+		t.p39[i] = crt_sym(crt_pow2_mod(39, m), m);
 		long long prod = 1;	/* m_0 ... m_{j-1} mod m_i */
 		for (int j = 0; j < CRT_MAXN; j++) {
 			t.pm[i][j] = j < i ? crt_sym(prod, m) : 0;
@@ -172,6 +176,52 @@ crt_residue(int h, int md, int lo, int p34, int p17, int m, float finv,
 {
 	return crt_smod25(CRT_MUL24(h, p34) + CRT_MUL24(md, p17) + lo, m, finv,
 	    rlo, rhi);
+}
+
+/*
+ * An integer-valued double x (|x| < 2^51) as four limbs of 13 bits:
+ * x = l[3] 2^39 + l[2] 2^26 + l[1] 2^13 + l[0] with |l[i]| <= 2^12.  Every
+ * step is exact in FP64.
+ */
+CRT_HD static inline void
+crt_limbs13(double x, int *l)
+{
+	const double t3 = rint(x * 0x1p-39);
+	const double r2 = x - t3 * 0x1p39;
+	const double t2 = rint(r2 * 0x1p-26);
+	const double r1 = r2 - t2 * 0x1p26;
+	const double t1 = rint(r1 * 0x1p-13);
+	l[3] = (int)t3;
+	l[2] = (int)t2;
+	l[1] = (int)t1;
+	l[0] = (int)(r1 - t1 * 0x1p13);
+}
+
+/*
+ * Residue modulo m_I of the integer with limbs l (crt_limbs13), in the
+ * symmetric range of m_I, in FP32 arithmetic that is exact: the limbs are
+ * given as floats (|l| <= 2^12), v = l0 + l1 c13 + l2 c26 + l3 c39 (c the
+ * symmetric 2^13k mod m, |c| <= 128) is an integer below 2^12 (1 + 3 * 128)
+ * < 2^21 at every step of its fused multiply-adds, and fl(v / m) is within
+ * |v| / m 2^-23 < 0.0009 of v / m, less than v / m's distance to a
+ * half-integer for odd m (at least 1 / (2m) > 0.0019): its rounding q is
+ * the nearest integer to v / m, and v - q m (exact in one fused
+ * multiply-add) lies in [-(m - 1) / 2, (m - 1) / 2] with no correction.
+ * m_0 = 256 divides 2^13: the residue is the low byte of l0, sign extended
+ * (l0i: l0 as an integer).  No integer multiply is left for the compiler to
+ * widen.
+ */
+template <int I>
+CRT_HD static inline int
+crt_residue13(const float *l, int l0i, const crt_tables &t)
+{
+	if (I == 0)
+		return (int)(int8_t)(uint8_t)(l0i & 0xff);
+	const float m = (float)CRT_MOD[I];
+	const float v = fmaf(l[3], (float)t.p39[I], fmaf(l[2], (float)t.p26[I],
+	    fmaf(l[1], (float)t.p13[I], l[0])));
+	const float q = rintf(v * t.finv[I]);
+	return (int)fmaf(-q, m, v);
 }
 
 /*

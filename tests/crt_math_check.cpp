@@ -1,6 +1,7 @@
 // CPU check of the CRT arithmetic in libxm_amd/csrc/crt_math.h against
-// exact 128-bit integer arithmetic: limb split and residues of integers up
-// to 2^50, and int32 residue-product sums (what the int8 matrix cores
+// exact 128-bit integer arithmetic: limb splits and residues of integers
+// up to 2^50 (17-bit limbs) and 2^51 (13-bit limbs, every quotient of the
+// range), and int32 residue-product sums (what the int8 matrix cores
 // compute per modulus) reconstructed by Garner's algorithm.
 #include "crt_math.h"
 
@@ -17,6 +18,53 @@ exact_mod(__int128 v, int m)
 	if (r < 0)
 		r += m;
 	return (int)(r >= (m + 1) / 2 ? r - m : r);
+}
+
+/* crt_residue13<I> on every v of its range, through limbs (l0 = v, the
+ * rest zero, covers v exactly when |v| <= 4096; larger v through l1), and
+ * on random integers of up to 51 bits split by crt_limbs13. */
+template <int I>
+static void
+check13(long &bad, std::mt19937_64 &rng)
+{
+	const int m = CRT_MOD[I];
+	const int V = 4096 * 385;
+	for (int v = -V; v <= V; v++) {
+		/* every v the limbs can give, as l0 = v (the function's FP32
+		 * steps on v itself; the I = 0 path reads l0 as an integer) */
+		const float lf[4] = {(float)v, 0.0f, 0.0f, 0.0f};
+		const int r = crt_residue13<I>(lf, v, T);
+		if (I != 0 && r != exact_mod(v, m))
+			bad++;
+	}
+	for (int it = 0; it < 300000; it++) {
+		long long x = (long long)(rng() >> 12) - (1ll << 51) + 1;
+		if (it < 8)
+			x = it < 4 ? (1ll << 51) - 1 - it : -(1ll << 51) + 1 + it;
+		int l[4];
+		crt_limbs13((double)x, l);
+		if ((__int128)l[3] * (1ll << 39) + (__int128)l[2] * (1ll << 26) +
+		    (__int128)l[1] * (1 << 13) + l[0] != x)
+			bad++;
+		for (int k = 0; k < 4; k++)
+			if (l[k] < -4096 || l[k] > 4096)
+				bad++;
+		const float lf[4] = {(float)l[0], (float)l[1], (float)l[2],
+		    (float)l[3]};
+		const int r = crt_residue13<I>(lf, l[0], T);
+		if (r != exact_mod(x, m))
+			bad++;
+	}
+}
+
+template <int I = 0>
+static void
+check13_all(long &bad, std::mt19937_64 &rng)
+{
+	if constexpr (I < CRT_MAXN) {
+		check13<I>(bad, rng);
+		check13_all<I + 1>(bad, rng);
+	}
 }
 
 int
@@ -86,6 +134,10 @@ main(void)
 				bad++;
 		}
 	}
+	// the 13-bit limb form: the FP32 quotient of every v the limbs can
+	// give (|v| <= 2^12 * 385) for every modulus, and residues of single
+	// integers up to 2^51, both ends included
+	check13_all(bad, rng);
 	// dot products of K integers of up to 2^49.5: int32 sums of residue
 	// products per modulus, then Garner
 	for (int n = 11; n <= CRT_MAXN; n++) {
