@@ -990,8 +990,8 @@ xcd_lists(const contract_plan *pl, uint32_t BP, uint32_t BQ,
 }
 
 /*
- * Grouped order (int8 CRT plans; XM_TILE_GROUPS=1 or 0 forces it on or off
- * for every plan): C blocks whose pairs read the same Y panels in the same
+ * Grouped order (XM_TILE_GROUPS=0 turns it off): C blocks whose pairs read
+ * the same Y panels in the same
  * order form a group, and the tiles of a group at one Q offset read the
  * same Y fragments step for step.  Items of a group's tiles at QCHUNK
  * consecutive Q offsets are dealt to the XCD lists longest first (each
@@ -999,9 +999,11 @@ xcd_lists(const contract_plan *pl, uint32_t BP, uint32_t BQ,
  * its tiles C block by C block, P tile by P tile: the workgroups an XCD
  * runs together share Y fragments across C blocks in its L2.  The one
  * workgroup a CU of the int8 kernel starts its tiles in step, and the
- * tiles of an item have equal K.  On the o=64 v=384 ladder the int8 GEMM
- * went from 11.1 to 10.6 ms; lists ordered by group instead of by length
- * ran 12.1-12.5 ms (their tails drained late).
+ * tiles of an item have equal K.  On the o=64 v=384 ladder the FP64 GEMM
+ * went from 18.0 to 17.1-17.5 ms, the BF16 split GEMM from 16.5 to 14.9-15.6
+ * and the int8 one from 11.1 to 10.4-10.6; lists ordered by group instead
+ * of by length ran 12.1-12.5 ms (int8: their tails drained late), and
+ * items of 2, 4 or 8 Q offsets ran no faster than 1.
  */
 #define QCHUNK 1
 
