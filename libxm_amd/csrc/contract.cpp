@@ -861,6 +861,8 @@ plan_release(contract_plan *pl)
 		HIP_OK("xm_contract", hipFree(pl->dev));
 	if (pl->d_ksw)
 		HIP_OK("xm_contract", hipFree(pl->d_ksw));
+	if (pl->d_queue)
+		HIP_OK("xm_contract", hipFree(pl->d_queue));
 	if (pl->ws && !pl->ws_external)
 		HIP_OK("xm_contract", hipFree(pl->ws));
 	if (pl->host_scalars)
@@ -957,6 +959,9 @@ plan_upload(contract_plan *pl)
 		    offs[3 + pl->ncand + c]);
 		nslot = std::max(nslot, pl->cand_fix[c].size());
 	}
+	/* counters of the persistent GEMM's tile queues, beside the K-split
+	 * sums */
+	HIP_OK(fn, hipMalloc((void **)&pl->d_queue, 64 * sizeof(uint32_t)));
 	if (nslot)
 		HIP_OK(fn, hipMalloc(&pl->d_ksw, nslot * 2 * KSPLIT_SLOT *
 		    sizeof(double)));

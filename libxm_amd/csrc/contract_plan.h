@@ -132,6 +132,7 @@ struct contract_plan {
 	gemm_tile *d_fix = nullptr;	/* the active candidate's */
 	size_t nfix = 0;
 	double *d_ksw = nullptr;	/* K-split partial sums */
+	uint32_t *d_queue = nullptr;	/* persistent GEMM's tile counters */
 	void *d_unfold = nullptr, *d_cin = nullptr, *d_cout = nullptr;
 	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 

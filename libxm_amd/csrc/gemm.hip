@@ -306,6 +306,51 @@ k_gemm_f64(const gemm_tile *__restrict__ tiles,
 			}
 }
 
+/* ---- persistent tile queues ---------------------------------------- */
+
+#define TILE_NONE 0xffffffffu
+#define NXCD_Q 8
+
+/*
+ * A launch's tiles as NXCD_Q lists (list x holds tiles[s * 8 + x], s <
+ * per: the order the launch-order kernels run them in) with one counter
+ * each, zeroed before the launch.  A workgroup takes the next tile of its
+ * own XCD's list (the XCC_ID hardware register) and, once that is empty,
+ * of the others in turn, so every list drains whatever XCD a workgroup
+ * runs on; each counter passes `per` at most once per workgroup.
+ */
+struct tile_queue {
+	uint32_t *ctr;
+	uint32_t per, x0, done;
+
+	__device__ tile_queue(uint32_t *c, uint32_t n) : ctr(c), per(n), done(0)
+	{
+		/* s_getreg_b32 hwreg(HW_REG_XCC_ID, 0, 4) */
+		x0 = (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) % NXCD_Q;
+	}
+	/* called by one thread of the workgroup */
+	__device__ uint32_t take()
+	{
+		for (uint32_t d = 0; d < NXCD_Q; d++) {
+			const uint32_t x = (x0 + d) % NXCD_Q;
+			if (done & (1u << x))
+				continue;
+			const uint32_t s = atomicAdd(&ctr[x], 1u);
+			if (s < per)
+				return s * NXCD_Q + x;
+			done |= 1u << x;
+		}
+		return TILE_NONE;
+	}
+};
+
+__global__ void
+k_queue_reset(uint32_t *__restrict__ ctr)
+{
+	if (threadIdx.x < NXCD_Q)
+		ctr[threadIdx.x] = 0;
+}
+
 /* ---- FP64, operands staged by DMA -------------------------------- */
 
 /*
@@ -341,11 +386,12 @@ struct fd_stage {
 	double y[FD_BK * FD_BQ];	/* [k][q] */
 };
 
-template <int FD_BK, int FD_NST, int MINW>
-__global__ void __launch_bounds__(GEMM_WG, MINW)
-k_gemm_f64_dma(const gemm_tile *__restrict__ tiles,
-    const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
-    const gemm_scalars *__restrict__ scp, double *__restrict__ ksw)
+/* One 64 x 128 tile on the workgroup's stages st. */
+template <int FD_BK, int FD_NST>
+__device__ __forceinline__ void
+f64_dma_tile(const gemm_tile tl, const gemm_cblock *__restrict__ cbs,
+    const gemm_pair *__restrict__ pairs, const gemm_scalars *__restrict__ scp,
+    double *__restrict__ ksw, fd_stage<FD_BK> *__restrict__ st)
 {
 	constexpr int MT = 2, NT = 4;
 	constexpr int NXD = FD_BK / 2, NYD = FD_BK;	/* X / Y DMAs per step */
@@ -353,9 +399,7 @@ k_gemm_f64_dma(const gemm_tile *__restrict__ tiles,
 	static_assert(NXD % 4 == 0 || NXD == 4, "X DMAs per wave");
 	constexpr int WAIT = (DPW * (FD_NST - 2)) | (0x7 << 4) | (0xf << 8);
 	typedef fd_stage<FD_BK> stage_t;
-	__shared__ stage_t st[FD_NST];
 
-	const gemm_tile tl = tiles[blockIdx.x];
 	if (tl.cblock == 0xffffffffu)
 		return;
 	const gemm_cblock cb = cbs[tl.cblock];
@@ -507,6 +551,41 @@ k_gemm_f64_dma(const gemm_tile *__restrict__ tiles,
 					C[o] = v;
 				}
 			}
+}
+
+/* One tile per workgroup, in launch order. */
+template <int FD_BK, int FD_NST, int MINW>
+__global__ void __launch_bounds__(GEMM_WG, MINW)
+k_gemm_f64_dma(const gemm_tile *__restrict__ tiles,
+    const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
+    const gemm_scalars *__restrict__ scp, double *__restrict__ ksw)
+{
+	__shared__ fd_stage<FD_BK> st[FD_NST];
+	f64_dma_tile<FD_BK, FD_NST>(tiles[blockIdx.x], cbs, pairs, scp, ksw, st);
+}
+
+/* Persistent form: a grid of resident workgroups takes tiles from the
+ * XCD lists until all are empty (tile_next). */
+template <int FD_BK, int FD_NST, int MINW>
+__global__ void __launch_bounds__(GEMM_WG, MINW)
+k_gemm_f64_dma_q(const gemm_tile *__restrict__ tiles, uint32_t per,
+    uint32_t *__restrict__ queue, const gemm_cblock *__restrict__ cbs,
+    const gemm_pair *__restrict__ pairs, const gemm_scalars *__restrict__ scp,
+    double *__restrict__ ksw)
+{
+	__shared__ fd_stage<FD_BK> st[FD_NST];
+	__shared__ uint32_t next;
+	tile_queue q(queue, per);
+	for (;;) {
+		if (threadIdx.x == 0)
+			next = q.take();
+		__syncthreads();
+		const uint32_t t = next;
+		__syncthreads();
+		if (t == TILE_NONE)
+			return;
+		f64_dma_tile<FD_BK, FD_NST>(tiles[t], cbs, pairs, scp, ksw, st);
+	}
 }
 
 /* ---- FP32 / complex / reference kernel ---------------------------- */
@@ -1393,6 +1472,35 @@ gemm_tile_record(contract_plan *pl, float gemm_ms)
 
 /* The kernel launch_gemm runs for the plan's active tile shape (the same
  * choices as below). */
+/*
+ * The persistent form of the FP64 DMA kernel (XM_GEMM_PERSIST=1; plans
+ * without queue counters, the no-allocation entry point's, never).  On the
+ * o=64 v=384 ladder it ran 17.9-18.0 ms against 17.2 for one workgroup per
+ * tile in launch order (same tiles, same occupancy of 3 workgroups a CU,
+ * whether a workgroup starts at its XCC_ID's list or at blockIdx % 8), so
+ * launch order stays the default.  Results are identical.
+ */
+static bool
+persistent(const contract_plan *pl)
+{
+	const char *e = getenv("XM_GEMM_PERSIST");
+	return e && *e == '1' && pl->d_queue != nullptr;
+}
+
+/* Workgroups of `kernel` (256 threads, no dynamic LDS) resident on the
+ * whole device at once. */
+static unsigned
+resident_wgs(const void *kernel)
+{
+	int per_cu = 0, ncu = 0, dev = 0;
+	HIP_OK("xm_contract", hipGetDevice(&dev));
+	HIP_OK("xm_contract", hipOccupancyMaxActiveBlocksPerMultiprocessor(
+	    &per_cu, kernel, GEMM_WG, 0));
+	HIP_OK("xm_contract", hipDeviceGetAttribute(&ncu,
+	    hipDeviceAttributeMultiprocessorCount, dev));
+	return (unsigned)std::max(1, per_cu * ncu);
+}
+
 static const char *
 gemm_kernel_name(const contract_plan *pl)
 {
@@ -1401,7 +1509,10 @@ gemm_kernel_name(const contract_plan *pl)
 		return "k_gemm_simple";
 	switch (pl->type) {
 	case XM_SCALAR_DOUBLE:
-		return q128 && pl->f64_dma ? "k_gemm_f64_dma" : "k_gemm_f64";
+		if (q128 && pl->f64_dma)
+			return persistent(pl) ? "k_gemm_f64_dma_q" :
+			    "k_gemm_f64_dma";
+		return "k_gemm_f64";
 	case XM_SCALAR_FLOAT:
 		if (pl->mfma_x4)
 			return "k_gemm_mfma_dw<x4>";
@@ -1425,7 +1536,15 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 		if (use_simple())
 			k_gemm_simple<double><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
-		else if (q128 && pl->f64_dma)
+		else if (q128 && pl->f64_dma && persistent(pl)) {
+			static const unsigned nwg = resident_wgs(
+			    (const void *)k_gemm_f64_dma_q<16, 2, 3>);
+			k_queue_reset<<<1, 64, 0, s>>>(pl->d_queue);
+			k_gemm_f64_dma_q<16, 2, 3><<<dim3(std::min<unsigned>(nwg,
+			    (unsigned)nt)), b, 0, s>>>(pl->d_tiles,
+			    (uint32_t)(nt / NXCD), pl->d_queue, pl->d_cblocks,
+			    pl->d_pairs, pl->d_scalars, pl->d_ksw);
+		} else if (q128 && pl->f64_dma)
 			k_gemm_f64_dma<16, 2, 3><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars, pl->d_ksw);
 		else if (q128)

@@ -6,9 +6,10 @@ On small contractions every C block of at least 64 steps is split, so
 these tests run the split path throughout.  Results must match the oracle
 to 1e-13 relative (the two partial sums round differently from one running
 sum), keep libxm's alpha == 0 and beta == 0 rules exactly, be bit-identical
-across tile shapes and between the DMA-staged and register-staged kernels
-(every shape cuts a C block at the same step), and XM_GEMM_KSPLIT=0 turns
-the split off.
+across tile shapes, between the DMA-staged and register-staged kernels and
+the DMA kernel's persistent form (XM_GEMM_PERSIST=1: resident workgroups
+taking tiles from per-XCD queues; every shape cuts a C block at the same
+step), and XM_GEMM_KSPLIT=0 turns the split off.
 """
 import os
 
@@ -37,7 +38,8 @@ def _workload():
 
 @pytest.fixture
 def env():
-    keys = ("XM_GEMM_KSPLIT", "XM_GEMM_TILE", "XM_GEMM_DMA")
+    keys = ("XM_GEMM_KSPLIT", "XM_GEMM_TILE", "XM_GEMM_DMA",
+            "XM_GEMM_PERSIST")
     old = {k: os.environ.get(k) for k in keys}
     yield os.environ
     for k, v in old.items():
@@ -114,13 +116,17 @@ def test_split_identical_across_kernels(gpulib, env):
     w = _workload()
     alloc = gpulib.allocator()
     out = {}
-    for name, tile, dma in (("64x128", "64x128", None),
-                            ("64x64", "64x64", None),
-                            ("64x128 registers", "64x128", "0")):
+    for name, tile, dma, kernel in (
+            ("64x128", "64x128", None, "k_gemm_f64_dma"),
+            ("64x128 persistent", "64x128", "persist", "k_gemm_f64_dma_q"),
+            ("64x64", "64x64", None, "k_gemm_f64"),
+            ("64x128 registers", "64x128", "0", "k_gemm_f64")):
         env["XM_GEMM_TILE"] = tile
-        if dma is None:
-            env.pop("XM_GEMM_DMA", None)
-        else:
+        env.pop("XM_GEMM_DMA", None)
+        env.pop("XM_GEMM_PERSIST", None)
+        if dma == "persist":
+            env["XM_GEMM_PERSIST"] = "1"
+        elif dma is not None:
             env["XM_GEMM_DMA"] = dma
         t, v, c = (s.build(gpulib, alloc) for s in w.specs)
         _fill(gpulib, w, (t, v, c), 51)
@@ -128,6 +134,7 @@ def test_split_identical_across_kernels(gpulib, env):
         st = gpulib.stats()
         assert st["ksplit_tiles"] > 0, (name, st)
         assert "%dx%d" % (st["tile_p"], st["tile_q"]) == tile
+        assert st["gemm_kernel"] == kernel, (name, st["gemm_kernel"])
         out[name] = w.c.read(gpulib, c)
         for x in (t, v, c):
             gpulib.tensor_free(x)
