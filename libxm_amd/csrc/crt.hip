@@ -72,8 +72,11 @@ struct crt_slab {
 /* A pair as the GEMM walks it (whole dwords: read with scalar loads). */
 struct crt_pair {
 	uint64_t xs, ys;		/* planes of the X and Y instances */
-	uint32_t nks, xrt, yrt;
+	uint32_t nsteps;		/* GEMM k steps (CR_KQ fragments each) */
+	uint32_t rtfr;			/* fragments of a row tile and modulus */
+	uint32_t xrt, yrt;
 	int32_t ratio;			/* multiply the sum by this first (1: no) */
+	uint32_t pad;
 };
 
 struct crt_cblock {
@@ -285,33 +288,43 @@ k_crt_quant(const crt_inst *__restrict__ insts, uint32_t ninst,
 /* ---- GEMM ------------------------------------------------------------ */
 
 /*
- * Workgroup tile 256 (P) x 128 (Q), 8 waves in 4 x 2 (two a SIMD), each
- * 64 x 64 as 2 x 2 MFMA tiles of 32 x 32 (64 int32 accumulators a lane);
- * one workgroup per CU.  With P up to 256 a tile covers a C block's rows,
- * so every Y fragment is read once per C block and modulus, and the C
- * block's tiles (one per 128 columns, placed on one XCD) share the X
- * fragments in L2.  A step covers 128 k: the workgroup stages 8 X and 4 Y
- * row tiles of four 1 KiB fragments each in LDS, every fragment one
- * wave-wide global_load_lds_dwordx4 (wave w moves the four quarters of X
- * row tile w and quarters 2 (w % 2) + {0, 1} of Y row tile w / 2: six a
- * step), and each wave runs 16 MFMAs on it, reading the next quarter's
- * fragments while the current quarter's MFMAs run.  CR_NST stages rotate
- * with the DMA CR_NST - 1 steps ahead of the MFMAs, one barrier per step.
- * The fetch walks the tile's (modulus, pair, k step) sequence flat, so the
- * prefetch crosses pair and modulus boundaries; the MFMAs walk it as
- * nested loops (the accumulators stay put in registers): where the factor
- * changes between pairs they are multiplied by the ratio (mod m), at the
- * end of each modulus reduced to residues and stored as bytes.  On the
- * o=64 v=384 ladder this ran 10.5 ms; 64 x 128 wave tiles (on 256 x 256
- * tiles, or in two 4-wave workgroups a CU) ran 10.7 and 11.3 ms, 128 x 128
- * wave tiles (one wave a SIMD) 12.7 ms: the barrier and LDS latencies had
- * no second wave to hide behind.
+ * Workgroup tile 256 (P) x 256 (Q), 8 waves in 4 x 2 (two a SIMD), each
+ * 64 x 128 as 2 x 4 MFMA tiles of 32 x 32 (128 int32 accumulators a
+ * lane); one workgroup per CU.  With P up to 256 a tile covers a C block's
+ * rows, so every Y fragment is read once per C block and modulus, and the
+ * C block's tiles (one per 256 columns, placed on one XCD) share the X
+ * fragments in L2.  A step covers CR_KQ 32-k fragments: the workgroup
+ * stages 8 X and 8 Y row tiles of CR_KQ 1 KiB fragments in LDS, every
+ * fragment one wave-wide global_load_lds_dwordx4 (wave w moves X row tile
+ * w and Y row tile w), and each wave runs 8 CR_KQ MFMAs on it, reading the
+ * next fragment's operands while the current one's MFMAs run.  CR_NST
+ * stages rotate with the DMA CR_NST - 1 steps ahead of the MFMAs, one
+ * barrier per step.  The fetch walks the tile's (modulus, pair, k step)
+ * sequence flat, so the prefetch crosses pair and modulus boundaries; the
+ * MFMAs walk it as nested loops (the accumulators stay put in registers):
+ * where the factor changes between pairs they are multiplied by the ratio
+ * (mod m), at the end of each modulus reduced to residues and stored as
+ * bytes.
+ *
+ * What bounds it is the operand stream into LDS, not the matrix cores: a
+ * model kernel with the same per-step work measured 11.3 TB/s of
+ * global_load_lds traffic (all CUs) from L2-resident data and 6.2 from
+ * HBM, while the ladder's 256 x 128 tiles need 29 TB/s at the int8 peak
+ * (170 int8 ops per staged byte).  256 x 256 tiles stage 1.5 times fewer
+ * bytes per op (256): on the o=64 v=384 ladder the GEMM and reconstruction
+ * went from 10.7 to 9.5 ms, although they pad Q = 576 to 768.  Tried and
+ * dropped on that ladder: 128-k steps in two stages (9.6 ms) and 32-k steps
+ * in eight (9.9); fragments of the next step read into a second register
+ * set during the current step's MFMAs (11.1: spills); waves whose columns
+ * all lie past the C block skipping their MFMAs (9.7); not fetching row
+ * tiles past the C block (10.6).  The earlier 256 x 128 form ran 10.7 ms,
+ * and 128 x 128 wave tiles (one wave a SIMD) 12.7.
  */
 #define CR_BP 256
-#define CR_BQ 128
+#define CR_BQ 256
 #define CR_WG 512
-#define CR_NST 3
-#define CR_KQ 4			/* 32-k fragments a step */
+#define CR_NST 4
+#define CR_KQ 2			/* 32-k fragments a step */
 
 typedef __attribute__((address_space(3))) void cr_lds_t;
 typedef __attribute__((address_space(1))) void cr_gbl_t;
@@ -320,9 +333,9 @@ typedef __attribute__((address_space(1))) void cr_gbl_t;
 static __device__ uint4 cr_zero_page[CR_KQ * 64];
 
 struct cr_stage {
-	/* X row tile j, quarter h at (4 j + h) * 64; Y row tile j at
-	 * (32 + 4 j + h) * 64 */
-	uint4 frag[48 * 64];
+	/* X row tile j, fragment h at (j CR_KQ + h) 64; Y row tile j at
+	 * ((8 + j) CR_KQ + h) 64 */
+	uint4 frag[16 * CR_KQ * 64];
 };
 
 __device__ __forceinline__ v4i
@@ -339,8 +352,8 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
     const crt_cblock *__restrict__ ccbs, const gemm_scalars *__restrict__ scp,
     int nmod)
 {
-	/* vmcnt = the DMAs of the CR_NST - 2 later steps (6 a step) */
-	constexpr int WAIT = ((CR_NST - 2) * 6) | (0x7 << 4) | (0xf << 8);
+	/* vmcnt = the DMAs of the CR_NST - 2 later steps (2 CR_KQ a step) */
+	constexpr int WAIT = ((CR_NST - 2) * 2 * CR_KQ) | (0x7 << 4) | (0xf << 8);
 	__shared__ cr_stage st[CR_NST];
 
 	const gemm_tile tl = tiles[blockIdx.x];
@@ -357,35 +370,32 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 	const int wp = wave & 3, wq = wave >> 2;
 	const uint32_t rx0 = tl.p0 / 32, ry0 = tl.q0 / 32;
 
-	v16i acc[2][2];
+	v16i acc[4][2];
 #pragma unroll
-	for (int a = 0; a < 2; a++)
+	for (int a = 0; a < 4; a++)
 #pragma unroll
 		for (int b = 0; b < 2; b++)
 			acc[a][b] = v16i{};
 
 	/* this wave's DMA sources for the fetch cursor's (modulus, pair):
-	 * step ks reads base + ks * stride + f * 64 + lane (uint4); X row
-	 * tile `wave` (quarters 0-3), Y row tile wave / 2 (quarters
-	 * 2 (wave % 2) + {0, 1}) */
+	 * step ks reads base + ks * stride + h * 64 + lane (uint4), X row
+	 * tile `wave` and Y row tile `wave` */
 	const uint4 *bx = cr_zero_page, *by = cr_zero_page;
 	uint32_t sx = 0, sy = 0;
-	const uint32_t yh = (wave & 1) * 128;
 	auto setup = [&](const crt_pair fp, uint32_t l) {
-		const uint32_t rtx = rx0 + wave, rty = ry0 + (wave >> 1);
+		const uint32_t rtx = rx0 + wave, rty = ry0 + wave;
 		bx = cr_zero_page;
 		sx = 0;
 		if (rtx < fp.xrt) {
 			bx = (const uint4 *)(uintptr_t)fp.xs +
-			    ((size_t)l * fp.xrt + rtx) * fp.nks * (CR_KQ * 64);
+			    ((size_t)l * fp.xrt + rtx) * fp.rtfr * 64;
 			sx = CR_KQ * 64;
 		}
-		by = cr_zero_page + yh;
+		by = cr_zero_page;
 		sy = 0;
 		if (rty < fp.yrt) {
 			by = (const uint4 *)(uintptr_t)fp.ys +
-			    ((size_t)l * fp.yrt + rty) * fp.nks * (CR_KQ * 64) +
-			    yh;
+			    ((size_t)l * fp.yrt + rty) * fp.rtfr * 64;
 			sy = CR_KQ * 64;
 		}
 	};
@@ -395,21 +405,19 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 #pragma unroll
 		for (int h = 0; h < CR_KQ; h++)
 			__builtin_amdgcn_global_load_lds((cr_gbl_t *)(a + 64 * h),
-			    (cr_lds_t *)&nx->frag[(4 * wave + h) * 64], 16, 0, 0);
+			    (cr_lds_t *)&nx->frag[(wave * CR_KQ + h) * 64], 16, 0, 0);
 #pragma unroll
-		for (int h = 0; h < 2; h++)
+		for (int h = 0; h < CR_KQ; h++)
 			__builtin_amdgcn_global_load_lds((cr_gbl_t *)(b + 64 * h),
-			    (cr_lds_t *)&nx->frag[(32 + 2 * wave + h) * 64], 16,
-			    0, 0);
+			    (cr_lds_t *)&nx->frag[((8 + wave) * CR_KQ + h) * 64],
+			    16, 0, 0);
 	};
-	/* fetch cursor over (modulus, pair, k step), CR_NST - 1 steps ahead
-	 * of the compute loops */
 	uint32_t fl = 0, fi = pbeg, fk = 0;
 	crt_pair fp = pairs[pbeg];
 	setup(fp, 0);
 	bool fmore = true;
 	auto fadvance = [&]() {
-		if (++fk < fp.nks)
+		if (++fk < fp.nsteps)
 			return;
 		fk = 0;
 		if (++fi == pend) {
@@ -433,26 +441,29 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 
 	uint8_t *cres = (uint8_t *)(uintptr_t)ccb.cres;
 	const size_t pq = (size_t)cb.p * cb.q;
-	/* residues of the finished modulus l */
 	auto store = [&](uint32_t l) {
 		const int m = CRT_MOD[l];
 		const uint32_t mg = d_ct.magic[l];
 		const int c31 = d_ct.c31[l], hi = d_ct.hi[l];
+		/* the extents made opaque here, so the store offsets are
+		 * computed at each store, not kept across the k loop */
+		uint32_t P = cb.p, Q = cb.q;
+		asm volatile("" : "+s"(P), "+s"(Q));
 		uint8_t *out = cres + l * pq;
 #pragma unroll
-		for (int a = 0; a < 2; a++)
+		for (int a = 0; a < 4; a++)
 #pragma unroll
 			for (int b = 0; b < 2; b++) {
 				const uint32_t p = tl.p0 + wp * 64 + 32 * b +
 				    (lane & 31);
 #pragma unroll
 				for (int i = 0; i < 16; i++) {
-					const uint32_t q = tl.q0 + wq * 64 + 32 * a +
+					const uint32_t q = tl.q0 + wq * 128 + 32 * a +
 					    8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
 					const int r = crt_smod(acc[a][b][i], m, mg, c31,
 					    hi);
-					if (p < cb.p && q < cb.q)
-						out[(size_t)q * cb.p + p] =
+					if (p < P && q < Q)
+						out[(size_t)q * P + p] =
 						    (uint8_t)(r & 0xff);
 				}
 				acc[a][b] = v16i{};
@@ -463,7 +474,7 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 		const uint32_t mg = d_ct.magic[l];
 		const int c31 = d_ct.c31[l], hi = d_ct.hi[l];
 #pragma unroll
-		for (int a = 0; a < 2; a++)
+		for (int a = 0; a < 4; a++)
 #pragma unroll
 			for (int b = 0; b < 2; b++)
 #pragma unroll
@@ -471,7 +482,6 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 					acc[a][b][i] = crt_smod(acc[a][b][i], m, mg,
 					    c31, hi) * ratio;
 	};
-	/* one 128-k step on stage cur, the fetch of a later one into nxt */
 	auto step = [&](const cr_stage *__restrict__ cur,
 	    cr_stage *__restrict__ nxt) {
 		const bool more = fmore;
@@ -480,41 +490,36 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 			fadvance();
 		}
 		/* D[q][p] += Y(q, k) X(p, k): A = Y fragment, B = X fragment;
-		 * quarter h + 1's fragments are read while quarter h's MFMAs
-		 * run */
-		v4i f[2][4];
-		auto rd = [&](v4i (&g)[4], int h) {
-			g[0] = frag4(cur->frag[(8 * wp + h) * 64 + lane]);
-			g[1] = frag4(cur->frag[(8 * wp + 4 + h) * 64 + lane]);
-			g[2] = frag4(cur->frag[(32 + 8 * wq + h) * 64 + lane]);
-			g[3] = frag4(cur->frag[(32 + 8 * wq + 4 + h) * 64 + lane]);
+		 * fragment h + 1's operands are read while h's MFMAs run */
+		v4i f[2][6];
+		auto rd = [&](v4i (&g)[6], int h) {
+			g[0] = frag4(cur->frag[((2 * wp) * CR_KQ + h) * 64 + lane]);
+			g[1] = frag4(cur->frag[((2 * wp + 1) * CR_KQ + h) * 64 + lane]);
+#pragma unroll
+			for (int a = 0; a < 4; a++)
+				g[2 + a] = frag4(cur->frag[((8 + 4 * wq + a) *
+				    CR_KQ + h) * 64 + lane]);
 		};
 		rd(f[0], 0);
 #pragma unroll
 		for (int h = 0; h < CR_KQ; h++) {
 			if (h + 1 < CR_KQ)
 				rd(f[(h + 1) & 1], h + 1);
-			const v4i (&g)[4] = f[h & 1];
-			acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(g[2], g[0], acc[0][0], 0, 0, 0);
-			acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(g[2], g[1], acc[0][1], 0, 0, 0);
-			acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(g[3], g[0], acc[1][0], 0, 0, 0);
-			acc[1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(g[3], g[1], acc[1][1], 0, 0, 0);
+			const v4i (&g)[6] = f[h & 1];
+#pragma unroll
+			for (int a = 0; a < 4; a++)
+#pragma unroll
+				for (int b = 0; b < 2; b++)
+					acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+					    g[2 + a], g[b], acc[a][b], 0, 0, 0);
 		}
-		/* schedule: the first quarter's reads, then per quarter the
-		 * next quarter's 4 LDS reads ahead of its own 4 MFMAs */
-		__builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+		__builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
 #pragma unroll
 		for (int h = 0; h < CR_KQ; h++) {
 			if (h + 1 < CR_KQ)
-				__builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
-			__builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+				__builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+			__builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
 		}
-		/*
-		 * The stage is read and this wave's DMAs of the next step have
-		 * landed (those just issued may stay in flight), then a bare
-		 * barrier publishes the stage (__syncthreads()' fence would
-		 * wait for the DMAs in flight).
-		 */
 		if (more)
 			__builtin_amdgcn_s_waitcnt(WAIT);
 		else
@@ -529,7 +534,7 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 			const crt_pair cp = pairs[i];
 			if (i != pbeg && cp.ratio != 1)
 				rescale(l, cp.ratio);
-			for (uint32_t ks = 0; ks < cp.nks; ks++) {
+			for (uint32_t ks = 0; ks < cp.nsteps; ks++) {
 				const int ns = cs + CR_NST - 1 < CR_NST ?
 				    cs + CR_NST - 1 : cs - 1;
 				step(&st[cs], &st[ns]);
@@ -537,8 +542,6 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 			}
 		}
 		store(l);
-		/* the stores complete before the next step's counted wait,
-		 * which would otherwise count them */
 		__builtin_amdgcn_s_waitcnt(0);
 	}
 }
@@ -741,7 +744,8 @@ crt_prepare(contract_plan *pl, const masks_t &mk, const xm_tensor_t *c)
 			memset(&cp, 0, sizeof cp);
 			cp.xs = ix;	/* instance ids until the planes exist */
 			cp.ys = iy;
-			cp.nks = (g.k + 127) / 128;	/* 128-k steps */
+			cp.nsteps = (g.k + 32 * CR_KQ - 1) / (32 * CR_KQ);
+			cp.rtfr = (g.k + 127) / 128 * 4;
 			cp.xrt = (cb.p + 31) / 32;
 			cp.yrt = (cb.q + 31) / 32;
 			cp.ratio = j == 0 ? 1 : (int32_t)std::lrint(
@@ -750,7 +754,7 @@ crt_prepare(contract_plan *pl, const masks_t &mk, const xm_tensor_t *c)
 			const double w = std::fabs(g.a_re / fn);
 			wx[ix] += w;
 			wy[iy] += w;
-			kt += cp.nks * 128.0;
+			kt += (g.k + 127) / 128 * 128.0;
 		}
 		for (auto &e : wx)
 			cs->wx = std::max(cs->wx, e.second);

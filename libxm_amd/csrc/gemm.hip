@@ -979,7 +979,7 @@ tile_candidates(const contract_plan *pl, std::vector<std::pair<uint32_t,
 		out.push_back(split_staged_kernel(pl->nslices) ?
 		    std::make_pair(128u, 128u) : std::make_pair(64u, 64u));
 	} else if (pl->precision == XM_PRECISION_SPLIT_INT8) {
-		out.push_back({256, 128});
+		out.push_back({256, 256});
 	} else if (pl->type == XM_SCALAR_DOUBLE && !use_simple()) {
 		const char *e = getenv("XM_GEMM_TILE");
 		if (e && strcmp(e, "64x64") == 0)
