@@ -615,6 +615,23 @@ def overflow_mode(lib, args):
         lib.lib.xm_allocator_deallocate(alloc, p)
 
 
+def stream_mode(args):
+    """The operand-stream model of the int8 GEMM step
+    (libxm_amd/stream_ceiling, built with the library): int8 TOP/s and
+    global_load_lds TB/s of its 256 x 128 and 256 x 256 tile steps with the
+    operands L2-resident, streamed from HBM, or not fetched."""
+    import subprocess
+    exe = os.path.join(ROOT, "libxm_amd", "stream_ceiling")
+    if not os.path.exists(exe):
+        raise RuntimeError("%s is not built (make -C libxm_amd/csrc)" % exe)
+    out = subprocess.run([exe], check=True, stdout=subprocess.PIPE,
+                         universal_newlines=True).stdout
+    for line in out.splitlines():
+        rec = json.loads(line)
+        rec["mode"] = "stream"
+        print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -628,7 +645,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--mode", choices=["contract", "copy", "ring", "precision",
-                                       "permute", "overflow", "backward"],
+                                       "permute", "overflow", "backward",
+                                       "stream"],
                     default="contract",
                     help="copy: xm_copy of a 4 GiB FP64 tensor, a plain "
                          "copy of known size (HBM counter calibration); "
@@ -638,7 +656,8 @@ def main():
                          "permute: xm_copy under six permutations of 4-D "
                          "tensors with blocks of 32, 16 and 4-28; overflow: "
                          "the ladder with v in pinned host memory; backward: "
-                         "xm_contract_backward of the ladder")
+                         "xm_contract_backward of the ladder; stream: "
+                         "the operand-stream model of the int8 GEMM step")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library runs on a single "
@@ -646,6 +665,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.mode == "stream":
+        return stream_mode(args)
     from libxm_amd import xm, workloads
     lib = xm.load()
     if args.mode == "copy":
