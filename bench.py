@@ -721,12 +721,15 @@ def main():
                      "peak": FP64_PEAK, "unit": "TFLOP/s",
                      "frac": F / (gemm_ms * 1e-3) / 1e12 / FP64_PEAK,
                      # the ladder's panels are all P-contiguous with even
-                     # extents: its 64 x 128 tiles run the DMA-staged kernel;
-                     # the tiles split in K are summed by k_ksplit_sum
-                     "traffic": stage_traffic(("k_gemm_f64_dma",
-                                               "k_ksplit_sum")),
+                     # extents: the DMA-staged kernels run it (the stacked
+                     # 128 x 128 form, or 64 x 128 tiles whose K-split
+                     # halves k_ksplit_sum adds)
+                     "traffic": stage_traffic((st["gemm_kernel"],
+                                               "k_ksplit_sum")
+                                              if st["ksplit_tiles"] else
+                                              (st["gemm_kernel"],)),
                      "traffic_source": TRAFFIC_SOURCE,
-                     "kernel": "k_gemm_f64_dma", "kernel_ms": gemm_ms,
+                     "kernel": st["gemm_kernel"], "kernel_ms": gemm_ms,
                      "ksplit_tiles": st["ksplit_tiles"],
                      "tile": [st["tile_p"], st["tile_q"]],
                      "tile_tuning_in_timed_steps": bool(st["tile_tuning"])},

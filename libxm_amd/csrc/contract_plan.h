@@ -65,6 +65,10 @@ struct contract_plan {
 	std::vector<gemm_tile> cand_fix[MAX_TILE_CANDIDATES];
 	gemm_tile *d_cand_fix[MAX_TILE_CANDIDATES] = {};
 	uint32_t cand_bp[MAX_TILE_CANDIDATES] = {}, cand_bq[MAX_TILE_CANDIDATES] = {};
+	/* candidate c is the stacked 128 x 128 form: cand_tiles[c] holds
+	 * gemm_stile records (three gemm_tile each) */
+	bool cand_stacked[MAX_TILE_CANDIDATES] = {};
+	bool stacked = false;		/* the active candidate's */
 	float cand_ms[MAX_TILE_CANDIDATES] = {};
 	int tile_choice = -1;		/* decided candidate, -1 while tuning */
 	int tile_next = 0;		/* calls of the tuning sequence so far */

@@ -588,6 +588,253 @@ k_gemm_f64_dma_q(const gemm_tile *__restrict__ tiles, uint32_t per,
 	}
 }
 
+/*
+ * Stacked form of the DMA kernel: 128 x 128 tiles (gemm_stile), 8 waves as
+ * 4 x 2 of 32 x 64, the same 16-k steps, DMAs and per-output k order as
+ * k_gemm_f64_dma, so results are identical to it without a K split.  The
+ * tile's rows are four 32-row tiles, each of its own C block: C blocks
+ * whose pairs read the same Y panels in the same order with the same k
+ * and factors (the ladder's C blocks of one v(a,b) block and one irrep of
+ * (i,j)) stack their rows, so a tile runs 128 rows without padding a
+ * C block of 144 or 400 rows to a multiple of 128, and stages 1.5 times
+ * fewer operand bytes per flop than 64 x 128.  Lane l moves rows 2l, 2l+1
+ * of an X column (row tile l / 16, whose C block's X panel for the
+ * current pair comes from four scalar loads per pair); a Y DMA moves a
+ * 128-row column as before.  Two workgroups a CU (64 KiB of stages each).
+ */
+#define FS_BP 128
+#define FS_WG 512
+
+template <int FD_BK>
+struct fs_stage {
+	double x[FD_BK * FS_BP];	/* [k][p] */
+	double y[FD_BK * FD_BQ];	/* [k][q] */
+};
+
+template <int FD_BK, int FD_NST, int MINW>
+__global__ void __launch_bounds__(FS_WG, MINW)
+k_gemm_f64_dma_s(const gemm_stile *__restrict__ tiles,
+    const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
+    const gemm_scalars *__restrict__ scp)
+{
+	constexpr int MT = 2, NT = 4;
+	constexpr int NXD = FD_BK, NYD = FD_BK;	/* one column per DMA */
+	constexpr int DPW = (NXD + NYD) / 8;	/* per wave */
+	static_assert(NXD % 8 == 0, "X DMAs per wave");
+	constexpr int WAIT = (DPW * (FD_NST - 2)) | (0x7 << 4) | (0xf << 8);
+	typedef fs_stage<FD_BK> stage_t;
+	__shared__ stage_t st[FD_NST];
+
+	const gemm_stile tl = tiles[blockIdx.x];
+	if (tl.head.cblock == 0xffffffffu)
+		return;
+	const gemm_cblock cb = cbs[tl.head.cblock];
+	const gemm_scalars sc = *scp;
+	const int tid = threadIdx.x, lane = tid & 63;
+	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+	const int wr = wave & 3;			/* this wave's row tile */
+	const int wp = wr * 32, wq = (wave >> 2) * 64;
+	const uint32_t q0 = tl.head.q0;
+	const uint32_t pend = cb.pair_end;
+	const uint32_t pbeg = (sc.flags & 1) ? pend : cb.pair_begin;
+
+	/* the row tiles' C blocks: first pair and row count (0: none; the
+	 * first pair is then the first member's, so loads stay in range) */
+	uint32_t mb[4], mp[4];
+#pragma unroll
+	for (int r = 0; r < 4; r++) {
+		const bool has = tl.cb[r] != TILE_ROWS_NONE;
+		const gemm_cblock m = cbs[has ? tl.cb[r] : tl.head.cblock];
+		mb[r] = m.pair_begin;
+		mp[r] = has ? m.p : 0;
+	}
+	/* (selects, not an indexed load: an array indexed at run time
+	 * would go to scratch memory) */
+	const uint32_t mcb = wr == 0 ? tl.cb[0] : wr == 1 ? tl.cb[1] :
+	    wr == 2 ? tl.cb[2] : tl.cb[3];
+	const uint32_t pr0 = wr == 0 ? tl.p0[0] : wr == 1 ? tl.p0[1] :
+	    wr == 2 ? tl.p0[2] : tl.p0[3];
+	const bool live = mcb != TILE_ROWS_NONE && q0 + wq < cb.q;
+	/* this lane's X rows in the DMAs: row tile lr, rows lp, lp + 1 of its
+	 * C block, which has lP rows */
+	const int lr = lane >> 4;
+	uint32_t lp = tl.p0[0], lP = mp[0];
+#pragma unroll
+	for (int r = 1; r < 4; r++) {
+		lp = lr == r ? tl.p0[r] : lp;
+		lP = lr == r ? mp[r] : lP;
+	}
+	lp += 2 * (lane & 15);
+
+	d4 acc[MT][NT];
+#pragma unroll
+	for (int i = 0; i < MT; i++)
+#pragma unroll
+		for (int j = 0; j < NT; j++)
+			acc[i][j] = d4{0, 0, 0, 0};
+
+	/* the lane's X panel and leading dimension for pair ri of the
+	 * C blocks' lists */
+	/* (four scalar loads and per-lane selects: a per-lane load would
+	 * count in vmcnt, and its wait would drain the DMAs in flight) */
+	auto xpanel = [&](uint32_t ri, uint64_t &x, uint32_t &ldx) {
+		uint32_t xl[4], xh[4], ld[4];
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const gemm_pair *g = pairs + mb[r] + ri;
+			xl[r] = __builtin_amdgcn_readfirstlane((uint32_t)g->x);
+			xh[r] = __builtin_amdgcn_readfirstlane(
+			    (uint32_t)(g->x >> 32));
+			ld[r] = __builtin_amdgcn_readfirstlane(g->ldx);
+		}
+		uint32_t lo = xl[0], hi = xh[0];
+		ldx = ld[0];
+#pragma unroll
+		for (int r = 1; r < 4; r++) {
+			lo = lr == r ? xl[r] : lo;
+			hi = lr == r ? xh[r] : hi;
+			ldx = lr == r ? ld[r] : ldx;
+		}
+		x = (uint64_t)hi << 32 | lo;
+	};
+	/* step at column k0 of pair px (X of this lane: fx, fldx) into
+	 * stage nx: wave w issues X and Y DMAs for columns w, w + 8 */
+	auto fetch = [&](stage_t *__restrict__ nx, const gemm_pair px,
+	    uint64_t fx, uint32_t fldx, uint32_t k0) {
+		const double *X = (const double *)(uintptr_t)fx;
+		const double *Y = (const double *)(uintptr_t)px.y;
+#pragma unroll
+		for (int d = 0; d < NXD / 8; d++) {
+			const int col = wave + 8 * d;
+			const uint32_t k = k0 + col;
+			const void *src = &fd_zero_page[lane];
+			if (k < px.k && lp < lP)
+				src = X + (size_t)lp + (size_t)k * fldx;
+			__builtin_amdgcn_global_load_lds((fd_gbl_t *)src,
+			    (fd_lds_t *)&nx->x[col * FS_BP], 16, 0, 0);
+		}
+#pragma unroll
+		for (int d = 0; d < NYD / 8; d++) {
+			const int col = wave + 8 * d;
+			const uint32_t k = k0 + col, q = q0 + 2 * lane;
+			const void *src = &fd_zero_page[lane];
+			if (k < px.k && q < cb.q)
+				src = Y + (size_t)q + (size_t)k * px.ldy;
+			__builtin_amdgcn_global_load_lds((fd_gbl_t *)src,
+			    (fd_lds_t *)&nx->y[col * FD_BQ], 16, 0, 0);
+		}
+	};
+
+	/* compute cursor (ip, kc, pr) with n steps left; fetch cursor
+	 * FD_NST - 1 steps ahead with fn left, its lane X in (fx, fldx) */
+	uint32_t ip = pbeg, kc = 0, n = pbeg == pend ? 0 : cb.steps;
+	uint32_t fip = ip, fk = 0, fn = n;
+	gemm_pair pr, fpr;
+	uint64_t fx = 0;
+	uint32_t fldx = 0;
+	if (n) {
+		pr = pairs[ip];
+		fpr = pr;
+		xpanel(0, fx, fldx);
+	}
+	auto fadvance = [&]() {
+		fk += FD_BK;
+		if (fk >= fpr.k) {
+			fk = 0;
+			if (++fip < pend) {
+				fpr = pairs[fip];
+				xpanel(fip - cb.pair_begin, fx, fldx);
+			}
+		}
+	};
+#pragma unroll
+	for (int s = 0; s < FD_NST - 1; s++)
+		if (fn) {
+			fetch(&st[s], fpr, fx, fldx, fk);
+			fadvance();
+			fn--;
+		}
+	__builtin_amdgcn_s_waitcnt(0);
+	__syncthreads();
+	auto step = [&](const stage_t *__restrict__ cur,
+	    stage_t *__restrict__ nxt) {
+		const bool more = fn != 0;
+		if (more) {
+			fetch(nxt, fpr, fx, fldx, fk);
+			fadvance();
+			fn--;
+		}
+		if (live) {
+			const double al = __dmul_rn(sc.alpha_re, pr.a_re);
+#pragma unroll
+			for (int kk = 0; kk < FD_BK; kk += 4) {
+				const int kr = kk + (lane >> 4), c16 = lane & 15;
+				double xa[MT], yb[NT];
+#pragma unroll
+				for (int i = 0; i < MT; i++)
+					xa[i] = cur->x[kr * FS_BP + wp + i * 16 + c16] * al;
+#pragma unroll
+				for (int j = 0; j < NT; j++)
+					yb[j] = cur->y[kr * FD_BQ + wq + j * 16 + c16];
+#pragma unroll
+				for (int i = 0; i < MT; i++)
+#pragma unroll
+					for (int j = 0; j < NT; j++)
+						acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+						    yb[j], xa[i], acc[i][j], 0, 0, 0);
+			}
+		}
+		if (more)
+			__builtin_amdgcn_s_waitcnt(WAIT);
+		else
+			__builtin_amdgcn_s_waitcnt(0);
+		__atomic_signal_fence(__ATOMIC_SEQ_CST);
+		__builtin_amdgcn_s_barrier();
+		__atomic_signal_fence(__ATOMIC_SEQ_CST);
+		kc += FD_BK;
+		if (kc >= pr.k) {
+			kc = 0;
+			if (++ip < pend)
+				pr = pairs[ip];
+		}
+	};
+	int cs = 0;
+	for (; n; n--) {
+		const int ns = cs + FD_NST - 1 < FD_NST ? cs + FD_NST - 1 : cs - 1;
+		step(&st[cs], &st[ns]);
+		cs = cs + 1 < FD_NST ? cs + 1 : 0;
+	}
+
+	/* this wave's rows: rows p0[wr] + wp' of C block mcb */
+	if (mcb == TILE_ROWS_NONE)
+		return;
+	const gemm_cblock mc = cbs[mcb];
+	double *C = (double *)(uintptr_t)mc.c;
+	const bool bz = sc.flags & 2;
+	const bool none = pbeg == pend;
+#pragma unroll
+	for (int i = 0; i < MT; i++)
+#pragma unroll
+		for (int j = 0; j < NT; j++)
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				uint32_t p = pr0 + i * 16 + (lane & 15);
+				uint32_t q = q0 + wq + j * 16 + 4 * r + (lane >> 4);
+				if (p < mc.p && q < mc.q) {
+					size_t o = (size_t)p + (size_t)q * mc.ldc;
+					double v;
+					if (bz)
+						v = none ? 0.0 : acc[i][j][r];
+					else if (none)
+						v = __dmul_rn(sc.beta_re, C[o]);
+					else
+						v = __dadd_rn(__dmul_rn(sc.beta_re, C[o]),
+						    acc[i][j][r]);
+					C[o] = v;
+				}
+			}
+}
+
 /* ---- FP32 / complex / reference kernel ---------------------------- */
 
 template <typename R> struct cx { R re, im; };
@@ -981,12 +1228,21 @@ tile_candidates(const contract_plan *pl, std::vector<std::pair<uint32_t,
 	} else if (pl->precision == XM_PRECISION_SPLIT_INT8) {
 		out.push_back({256, 256});
 	} else if (pl->type == XM_SCALAR_DOUBLE && !use_simple()) {
+		/* the stacked 128 x 128 form where the DMA kernel can run,
+		 * then 64 x 128 (the no-allocation entry point's plans take the
+		 * first candidate and have no storage for stacked tiles) */
 		const char *e = getenv("XM_GEMM_TILE");
+		const bool st = pl->f64_dma && !pl->single_tile;
 		if (e && strcmp(e, "64x64") == 0)
 			out.push_back({64, 64});
 		else if (e && strcmp(e, "64x128") == 0)
 			out.push_back({64, 128});
-		else {
+		else if (e && strcmp(e, "128x128") == 0 && st)
+			out.push_back({128, 128});
+		else if (st) {
+			out.push_back({128, 128});
+			out.push_back({64, 128});
+		} else {
 			out.push_back({64, 128});
 			out.push_back({64, 64});
 		}
@@ -1197,6 +1453,101 @@ order_tiles_grouped(const contract_plan *pl, uint32_t BP, uint32_t BQ,
 }
 
 /*
+ * Tiles of the stacked form (gemm_stile, three gemm_tile records each):
+ * C blocks with the same Q and the same pair sequence on the Y side (Y
+ * panel, its leading dimension, k and factor at every position) form a
+ * stack; its rows are the members' rows in 32-row tiles (a member of P
+ * rows takes ceil(P / 32)), four row tiles to a tile.  Items (a stack's
+ * tiles at one Q offset) are dealt to the XCD lists longest first, as in
+ * the grouped order, so the workgroups an XCD runs together read the same
+ * Y fragments.
+ */
+static void
+order_tiles_stacked(const contract_plan *pl, std::vector<gemm_tile> &out)
+{
+	const uint32_t BQ = 128;
+	std::map<std::vector<uint64_t>, uint32_t> sid;
+	std::vector<std::vector<uint32_t>> stacks;
+	for (uint32_t i = 0; i < pl->cblocks.size(); i++) {
+		const gemm_cblock &cb = pl->cblocks[i];
+		std::vector<uint64_t> key = {cb.q, cb.pair_end - cb.pair_begin};
+		for (uint32_t p = cb.pair_begin; p < cb.pair_end; p++) {
+			const gemm_pair &g = pl->pairs[p];
+			uint64_t re, im;
+			memcpy(&re, &g.a_re, 8);
+			memcpy(&im, &g.a_im, 8);
+			key.insert(key.end(), {g.y, g.ldy, g.k, re, im});
+		}
+		if (cb.pair_begin == cb.pair_end)
+			key.push_back(0x8000000000000000ull | i);	/* alone */
+		auto it = sid.find(key);
+		if (it == sid.end()) {
+			it = sid.emplace(key, (uint32_t)stacks.size()).first;
+			stacks.emplace_back();
+		}
+		stacks[it->second].push_back(i);
+	}
+	struct item {
+		double work;
+		uint32_t s, q0;
+	};
+	std::vector<item> items;
+	std::vector<std::vector<std::pair<uint32_t, uint32_t>>> rows(
+	    stacks.size());
+	for (uint32_t s = 0; s < stacks.size(); s++) {
+		for (uint32_t i : stacks[s])
+			for (uint32_t p0 = 0; p0 < pl->cblocks[i].p; p0 += 32)
+				rows[s].push_back({i, p0});
+		const gemm_cblock &c0 = pl->cblocks[stacks[s][0]];
+		double kt = 1;
+		for (uint32_t p = c0.pair_begin; p < c0.pair_end; p++)
+			kt += pl->pairs[p].k;
+		const double nt = (double)((rows[s].size() + 3) / 4);
+		for (uint32_t q0 = 0; q0 < c0.q; q0 += BQ)
+			items.push_back({kt * nt, s, q0});
+	}
+	std::vector<size_t> order(items.size());
+	for (size_t i = 0; i < order.size(); i++)
+		order[i] = i;
+	std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+		return items[a].work > items[b].work;
+	});
+	double load[NXCD] = {0};
+	std::vector<gemm_stile> bins[NXCD];
+	for (size_t o : order) {
+		const int x = (int)(std::min_element(load, load + NXCD) - load);
+		const item &it = items[o];
+		load[x] += it.work;
+		const auto &rw = rows[it.s];
+		for (size_t r0 = 0; r0 < rw.size(); r0 += 4) {
+			gemm_stile t;
+			t.head = {stacks[it.s][0], 0, it.q0, 0};
+			for (size_t r = 0; r < 4; r++) {
+				t.cb[r] = r0 + r < rw.size() ? rw[r0 + r].first :
+				    TILE_ROWS_NONE;
+				t.p0[r] = r0 + r < rw.size() ? rw[r0 + r].second : 0;
+			}
+			bins[x].push_back(t);
+		}
+	}
+	size_t per = 0;
+	for (auto &b : bins)
+		per = std::max(per, b.size());
+	static_assert(sizeof(gemm_stile) == 3 * sizeof(gemm_tile),
+	    "a stacked tile is three tile records");
+	gemm_stile none;
+	memset(&none, 0, sizeof none);
+	none.head.cblock = 0xffffffffu;
+	std::vector<gemm_stile> flat(per * NXCD, none);
+	for (int x = 0; x < NXCD; x++)
+		for (size_t s = 0; s < bins[x].size(); s++)
+			flat[s * NXCD + x] = bins[x][s];
+	out.resize(flat.size() * 3);
+	if (!flat.empty())
+		memcpy(out.data(), flat.data(), flat.size() * sizeof(gemm_stile));
+}
+
+/*
  * K split of FP64 tiles (not with XM_GEMM_KSPLIT=0 or the FMA kernel).
  * An XCD's list ends when its last tiles finish, and with 64 x 128 tiles
  * of up to 18432 k (the o=64 v=384 ladder: 4470 tiles on 768 slots) the
@@ -1240,6 +1591,9 @@ mark_ksplit(contract_plan *pl)
 	}
 	if (!ksplit_enabled(pl))
 		return;
+	for (int c = 0; c < pl->ncand; c++)
+		if (pl->cand_stacked[c])
+			return;		/* every candidate runs all of K */
 	size_t ntiles = 0;
 	for (const gemm_cblock &cb : pl->cblocks)
 		ntiles += (size_t)((cb.p + BP - 1) / BP) * ((cb.q + BQ - 1) / BQ);
@@ -1406,11 +1760,17 @@ build_tiles(contract_plan *pl)
 	std::vector<std::pair<uint32_t, uint32_t>> cand;
 	tile_candidates(pl, cand);
 	pl->ncand = (int)std::min<size_t>(cand.size(), MAX_TILE_CANDIDATES);
+	for (int c = 0; c < pl->ncand; c++)
+		pl->cand_stacked[c] = pl->type == XM_SCALAR_DOUBLE &&
+		    pl->precision == XM_PRECISION_NATIVE && cand[c].first == 128;
 	mark_ksplit(pl);
 	for (int c = 0; c < pl->ncand; c++) {
 		pl->cand_bp[c] = cand[c].first;
 		pl->cand_bq[c] = cand[c].second;
-		if (grouped_order(pl))
+		if (pl->cand_stacked[c]) {
+			order_tiles_stacked(pl, pl->cand_tiles[c]);
+			pl->cand_fix[c].clear();
+		} else if (grouped_order(pl))
 			order_tiles_grouped(pl, cand[c].first, cand[c].second,
 			    pl->cand_tiles[c], pl->cand_fix[c]);
 		else
@@ -1433,6 +1793,7 @@ select_tiles(contract_plan *pl)
 {
 	int c = pl->tile_choice >= 0 ? pl->tile_choice : 0;
 	pl->tile_active = c;
+	pl->stacked = pl->cand_stacked[c];
 	pl->tiles = pl->cand_tiles[c];
 	pl->d_tiles = pl->d_cand_tiles[c];
 	pl->d_fix = pl->d_cand_fix[c];
@@ -1454,6 +1815,7 @@ gemm_tile_record(contract_plan *pl, float gemm_ms)
 	if (++pl->tile_next <= pl->ncand) {
 		const int c = pl->tile_next - 1;
 		pl->tile_active = c;
+		pl->stacked = pl->cand_stacked[c];
 		pl->tiles = pl->cand_tiles[c];
 		pl->d_tiles = pl->d_cand_tiles[c];
 		pl->d_fix = pl->d_cand_fix[c];
@@ -1509,6 +1871,8 @@ gemm_kernel_name(const contract_plan *pl)
 		return "k_gemm_simple";
 	switch (pl->type) {
 	case XM_SCALAR_DOUBLE:
+		if (pl->stacked)
+			return "k_gemm_f64_dma_s";
 		if (q128 && pl->f64_dma)
 			return persistent(pl) ? "k_gemm_f64_dma_q" :
 			    "k_gemm_f64_dma";
@@ -1535,6 +1899,10 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 		const bool q128 = pl->cand_bq[pl->tile_active] == 128;
 		if (use_simple())
 			k_gemm_simple<double><<<g, b, 0, s>>>(pl->d_tiles,
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+		else if (pl->stacked)
+			k_gemm_f64_dma_s<16, 2, 2><<<dim3((unsigned)(nt / 3)),
+			    dim3(FS_WG), 0, s>>>((const gemm_stile *)pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		else if (q128 && pl->f64_dma && persistent(pl)) {
 			static const unsigned nwg = resident_wgs(

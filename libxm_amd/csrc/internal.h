@@ -299,6 +299,21 @@ struct gemm_tile {
 	uint32_t ksplit;	/* 0: all of K; else 1 + 2 * slot + half */
 };
 
+/*
+ * A 128 x 128 tile of stacked C blocks (FP64): C blocks whose pairs read
+ * the same Y panels with the same extents and factors share the tile's
+ * columns, their rows stacked in 32-row tiles.  Row tile r of the tile
+ * holds rows p0[r] .. p0[r] + 31 of C block cb[r] (cb[r] == TILE_ROWS_NONE:
+ * no rows).  head.cblock is the first member (its pairs give Y, k and the
+ * factors), head.q0 the first column; head.p0 and head.ksplit are 0.  A
+ * candidate's tile list holds one as three consecutive gemm_tile records.
+ */
+struct gemm_stile {
+	gemm_tile head;
+	uint32_t cb[4], p0[4];
+};
+#define TILE_ROWS_NONE 0xffffffffu
+
 /* Scalars of one contraction call, read by the kernels from device
  * memory so a captured launch sequence can be replayed. */
 struct gemm_scalars {

@@ -1,7 +1,8 @@
 """Tile-shape autotuning of the FP64 grouped GEMM.
 
-A plan's first calls each time one tile-shape candidate (64 x 128, then
-64 x 64); later calls use the faster.  Every candidate must give
+A plan's first calls each time one tile-shape candidate (plans the DMA
+kernel can run: the stacked 128 x 128 form, then 64 x 128; others: 64 x 128,
+then 64 x 64); later calls use the faster.  Every candidate must give
 bit-identical results (each output accumulates over the same k sequence in
 the same order), a new plan whose C blocks have the same block-size
 histogram starts with the decision, and XM_GEMM_TILE pins the shape.
@@ -29,11 +30,12 @@ def _run(lib, w, t, v, c):
     return st, w.c.read(lib, c)
 
 
-@pytest.mark.parametrize("make", [
-    lambda: workloads.ladder_d2h(32, 96, 3),
-    lambda: workloads.ring_d2h(24, 72, 5, lo=4, hi=12),
+@pytest.mark.parametrize("make,cands", [
+    (lambda: workloads.ladder_d2h(32, 96, 3), [(128, 128), (64, 128)]),
+    (lambda: workloads.ring_d2h(24, 72, 5, lo=4, hi=12),
+     [(64, 128), (64, 64)]),
 ], ids=["ladder", "ring"])
-def test_tuning_sequence_and_identical_results(gpulib, make):
+def test_tuning_sequence_and_identical_results(gpulib, make, cands):
     os.environ.pop("XM_GEMM_TILE", None)
     w = make()
     alloc = gpulib.allocator()
@@ -47,8 +49,8 @@ def test_tuning_sequence_and_identical_results(gpulib, make):
         # a fresh signature: candidate 0 warms untimed, each candidate is
         # timed once, then the decision
         assert tuning == [1, 1, 1, 0], tuning
-        assert shapes[:3] == [(64, 128), (64, 128), (64, 64)], shapes
-    assert shapes[3] in ((64, 128), (64, 64))
+        assert shapes[:3] == [cands[0], cands[0], cands[1]], shapes
+    assert shapes[3] in cands
     r1 = runs[0][1]
     for _, r in runs[1:]:
         for k in r1:
@@ -68,23 +70,31 @@ def test_tuning_sequence_and_identical_results(gpulib, make):
 
 
 def test_pinned_tile_shape(gpulib):
+    """Each pinned shape runs alone; with the K split off (which the
+    stacked form never uses) all three give the same bits."""
     w = workloads.ladder_d2h(32, 96, 4)
     alloc = gpulib.allocator()
     out = {}
+    kernels = {"64x64": "k_gemm_f64", "64x128": "k_gemm_f64_dma",
+               "128x128": "k_gemm_f64_dma_s"}
     try:
-        for shape in ("64x64", "64x128"):
+        os.environ["XM_GEMM_KSPLIT"] = "0"
+        for shape in ("64x64", "64x128", "128x128"):
             os.environ["XM_GEMM_TILE"] = shape
             t, v, c = (s.build(gpulib, alloc) for s in w.specs)
             _fill(gpulib, w, (t, v, c), 21)
             st, r = _run(gpulib, w, t, v, c)
             assert st["tile_tuning"] == 0
             assert "%dx%d" % (st["tile_p"], st["tile_q"]) == shape
+            assert st["gemm_kernel"] == kernels[shape], st["gemm_kernel"]
             out[shape] = r
             for x in (t, v, c):
                 gpulib.tensor_free(x)
     finally:
         os.environ.pop("XM_GEMM_TILE", None)
-    for k in out["64x64"]:
-        assert np.array_equal(np.asarray(out["64x64"][k]),
-                              np.asarray(out["64x128"][k]))
+        os.environ.pop("XM_GEMM_KSPLIT", None)
+    for shape in ("64x128", "128x128"):
+        for k in out["64x64"]:
+            assert np.array_equal(np.asarray(out["64x64"][k]),
+                                  np.asarray(out[shape][k])), (shape, k)
     gpulib.allocator_destroy(alloc)

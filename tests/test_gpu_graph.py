@@ -7,6 +7,8 @@ and eager work (block writes, xm_add) runs between the replays.  After
 every call the results must be bit-identical, and once a plan's tile shape
 is decided its calls must replay the graph.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -48,22 +50,28 @@ def _sequence(lib, w, seed, graph, splits=None):
     return outs, replayed
 
 
-@pytest.mark.parametrize("make", [
-    lambda: workloads.ladder_d2h(32, 96, 3),
-    lambda: workloads.ring_d2h(24, 72, 5, lo=4, hi=12),
-], ids=["ladder", "ring"])
-def test_graph_replays_match_direct_launches(gpulib, make):
+@pytest.mark.parametrize("make,tile", [
+    (lambda: workloads.ladder_d2h(32, 96, 3), None),
+    (lambda: workloads.ladder_d2h(32, 96, 3), "64x128"),
+    (lambda: workloads.ring_d2h(24, 72, 5, lo=4, hi=12), None),
+], ids=["ladder", "ladder-64x128", "ring"])
+def test_graph_replays_match_direct_launches(gpulib, make, tile):
     w = make()
-    direct, r0 = _sequence(gpulib, w, 31, False)
-    splits = []
-    graph, r1 = _sequence(gpulib, w, 31, True, splits)
+    if tile:
+        os.environ["XM_GEMM_TILE"] = tile
+    try:
+        direct, r0 = _sequence(gpulib, w, 31, False)
+        splits = []
+        graph, r1 = _sequence(gpulib, w, 31, True, splits)
+    finally:
+        os.environ.pop("XM_GEMM_TILE", None)
     assert not any(r0)
-    # the ladder's replays include the K-split sum kernel
-    if w.name.startswith("ladder"):
+    # the 64 x 128 ladder's replays include the K-split sum kernel
+    if tile:
         assert all(splits), splits
-    # calls 0-2 tune the tile shape (direct launches); every later call of
-    # the plan replays a graph
-    assert all(r1[3:]), r1
+    # calls 0-2 tune the tile shape (direct launches; a pinned shape is
+    # decided at once); every later call of the plan replays a graph
+    assert all(r1[1 if tile else 3:]), r1
     for n, (a, b) in enumerate(zip(direct, graph)):
         for k in a:
             assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), (n, k)
