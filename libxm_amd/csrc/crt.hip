@@ -290,10 +290,13 @@ k_crt_quant(const crt_inst *__restrict__ insts, uint32_t ninst,
 /*
  * Workgroup tile 256 (P) x 256 (Q), 8 waves in 4 x 2 (two a SIMD), each
  * 64 x 128 as 2 x 4 MFMA tiles of 32 x 32 (128 int32 accumulators a
- * lane); one workgroup per CU.  With P up to 256 a tile covers a C block's
- * rows, so every Y fragment is read once per C block and modulus, and the
- * C block's tiles (one per 256 columns, placed on one XCD) share the X
- * fragments in L2.  A step covers CR_KQ 32-k fragments: the workgroup
+ * lane); one workgroup per CU.  A tile record (gemm_stile8) names the
+ * C block and first row of each of its eight 32-row tiles: the rows of one
+ * C block (by default), or of the C blocks of a group that read the same
+ * Y fragments, stacked (XM_CRT_STACK=1).  With P up to 256 a tile covers a
+ * C block's rows, so every Y fragment is read once per C block and
+ * modulus, and the C block's tiles (one per 256 columns, placed on one
+ * XCD) share the X fragments in L2.  A step covers CR_KQ 32-k fragments: the workgroup
  * stages 8 X and 8 Y row tiles of CR_KQ 1 KiB fragments in LDS, every
  * fragment one wave-wide global_load_lds_dwordx4 (wave w moves X row tile
  * w and Y row tile w), and each wave runs 8 CR_KQ MFMAs on it, reading the
@@ -347,7 +350,7 @@ frag4(uint4 v)
 }
 
 __global__ void __launch_bounds__(CR_WG, 1)
-k_gemm_crt(const gemm_tile *__restrict__ tiles,
+k_gemm_crt(const gemm_stile8 *__restrict__ tiles,
     const gemm_cblock *__restrict__ cbs, const crt_pair *__restrict__ pairs,
     const crt_cblock *__restrict__ ccbs, const gemm_scalars *__restrict__ scp,
     int nmod)
@@ -356,19 +359,28 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 	constexpr int WAIT = ((CR_NST - 2) * 2 * CR_KQ) | (0x7 << 4) | (0xf << 8);
 	__shared__ cr_stage st[CR_NST];
 
-	const gemm_tile tl = tiles[blockIdx.x];
+	const gemm_stile8 *tp = tiles + blockIdx.x;
+	const gemm_tile tl = tp->head;
 	if (tl.cblock == 0xffffffffu)
 		return;
 	const gemm_scalars sc = *scp;
+	/* the first C block of the tile's stack: its pairs give Y, k steps
+	 * and factor ratios, shared by every C block of the stack */
 	const gemm_cblock cb = cbs[tl.cblock];
 	const uint32_t pbeg = cb.pair_begin, pend = cb.pair_end;
 	if ((sc.flags & 1) || pbeg == pend)
 		return;		/* C = beta C: the reconstruction alone */
-	const crt_cblock ccb = ccbs[tl.cblock];
 	const int tid = threadIdx.x, lane = tid & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const int wp = wave & 3, wq = wave >> 2;
-	const uint32_t rx0 = tl.p0 / 32, ry0 = tl.q0 / 32;
+	const uint32_t ry0 = tl.q0 / 32;
+	/* X row tile `wave` of the tile: row tile xrt0 of C block xcb (its
+	 * pairs from xpb on), none if xcb is TILE_ROWS_NONE (loads at
+	 * wave-uniform addresses: scalar) */
+	const uint32_t xcb = tp->cb[wave];
+	const bool xhas = xcb != TILE_ROWS_NONE;
+	const uint32_t xrt0 = tp->p0[wave] / 32;
+	const uint32_t xpb = xhas ? cbs[xcb].pair_begin : pbeg;
 
 	v16i acc[4][2];
 #pragma unroll
@@ -382,13 +394,16 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 	 * tile `wave` and Y row tile `wave` */
 	const uint4 *bx = cr_zero_page, *by = cr_zero_page;
 	uint32_t sx = 0, sy = 0;
-	auto setup = [&](const crt_pair fp, uint32_t l) {
-		const uint32_t rtx = rx0 + wave, rty = ry0 + wave;
+	/* pair i of the stack: Y from the first C block's pair fp, X from
+	 * this wave's C block's pair i */
+	auto setup = [&](const crt_pair fp, uint32_t l, uint32_t i) {
+		const uint32_t rty = ry0 + wave;
 		bx = cr_zero_page;
 		sx = 0;
-		if (rtx < fp.xrt) {
-			bx = (const uint4 *)(uintptr_t)fp.xs +
-			    ((size_t)l * fp.xrt + rtx) * fp.rtfr * 64;
+		if (xhas) {
+			const crt_pair xp = pairs[xpb + (i - pbeg)];
+			bx = (const uint4 *)(uintptr_t)xp.xs +
+			    ((size_t)l * xp.xrt + xrt0) * xp.rtfr * 64;
 			sx = CR_KQ * 64;
 		}
 		by = cr_zero_page;
@@ -414,7 +429,7 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 	};
 	uint32_t fl = 0, fi = pbeg, fk = 0;
 	crt_pair fp = pairs[pbeg];
-	setup(fp, 0);
+	setup(fp, 0, pbeg);
 	bool fmore = true;
 	auto fadvance = [&]() {
 		if (++fk < fp.nsteps)
@@ -428,7 +443,7 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 			}
 		}
 		fp = pairs[fi];
-		setup(fp, fl);
+		setup(fp, fl, fi);
 	};
 #pragma unroll
 	for (int s = 0; s < CR_NST - 1; s++)
@@ -439,35 +454,44 @@ k_gemm_crt(const gemm_tile *__restrict__ tiles,
 	__builtin_amdgcn_s_waitcnt(0);
 	__syncthreads();
 
-	uint8_t *cres = (uint8_t *)(uintptr_t)ccb.cres;
-	const size_t pq = (size_t)cb.p * cb.q;
+	/* residues of the finished modulus l: row tile 2 wp + b of the tile
+	 * is rows p0 .. p0 + 31 of its own C block */
 	auto store = [&](uint32_t l) {
 		const int m = CRT_MOD[l];
 		const uint32_t mg = d_ct.magic[l];
 		const int c31 = d_ct.c31[l], hi = d_ct.hi[l];
-		/* the extents made opaque here, so the store offsets are
-		 * computed at each store, not kept across the k loop */
-		uint32_t P = cb.p, Q = cb.q;
-		asm volatile("" : "+s"(P), "+s"(Q));
-		uint8_t *out = cres + l * pq;
 #pragma unroll
-		for (int a = 0; a < 4; a++)
-#pragma unroll
-			for (int b = 0; b < 2; b++) {
-				const uint32_t p = tl.p0 + wp * 64 + 32 * b +
+		for (int b = 0; b < 2; b++) {
+			const uint32_t mcb = tp->cb[2 * wp + b];
+			if (mcb != TILE_ROWS_NONE) {
+				const crt_cblock mc = ccbs[mcb];
+				/* the extents made opaque here, so the store
+				 * offsets are computed at each store, not kept
+				 * across the k loop */
+				uint32_t P = mc.p, Q = mc.q;
+				asm volatile("" : "+s"(P), "+s"(Q));
+				uint8_t *out = (uint8_t *)(uintptr_t)mc.cres +
+				    l * (size_t)P * Q;
+				const uint32_t p = tp->p0[2 * wp + b] +
 				    (lane & 31);
 #pragma unroll
-				for (int i = 0; i < 16; i++) {
-					const uint32_t q = tl.q0 + wq * 128 + 32 * a +
-					    8 * (i / 4) + 4 * (lane >> 5) + (i % 4);
-					const int r = crt_smod(acc[a][b][i], m, mg, c31,
-					    hi);
-					if (p < P && q < Q)
-						out[(size_t)q * P + p] =
-						    (uint8_t)(r & 0xff);
-				}
-				acc[a][b] = v16i{};
+				for (int a = 0; a < 4; a++)
+#pragma unroll
+					for (int i = 0; i < 16; i++) {
+						const uint32_t q = tl.q0 + wq * 128 +
+						    32 * a + 8 * (i / 4) +
+						    4 * (lane >> 5) + (i % 4);
+						const int r = crt_smod(acc[a][b][i], m,
+						    mg, c31, hi);
+						if (p < P && q < Q)
+							out[(size_t)q * P + p] =
+							    (uint8_t)(r & 0xff);
+					}
 			}
+#pragma unroll
+			for (int a = 0; a < 4; a++)
+				acc[a][b] = v16i{};
+		}
 	};
 	auto rescale = [&](uint32_t l, int ratio) {
 		const int m = CRT_MOD[l];
@@ -931,9 +955,9 @@ crt_gemm(contract_plan *pl, hipStream_t s)
 		return;
 	const size_t nt = pl->tiles.size();
 	if (nt) {
-		k_gemm_crt<<<dim3((unsigned)nt), dim3(CR_WG), 0, s>>>(pl->d_tiles,
-		    pl->d_cblocks, cs->d_pairs, cs->d_cbs, pl->d_scalars,
-		    cs->nmod);
+		k_gemm_crt<<<dim3((unsigned)(nt / 5)), dim3(CR_WG), 0, s>>>(
+		    (const gemm_stile8 *)pl->d_tiles, pl->d_cblocks,
+		    cs->d_pairs, cs->d_cbs, pl->d_scalars, cs->nmod);
 		check("CRT GEMM");
 		pl->gemm_launches = 1;
 		pl->gemm_kernel = "k_gemm_crt";

@@ -1453,21 +1453,24 @@ order_tiles_grouped(const contract_plan *pl, uint32_t BP, uint32_t BQ,
 }
 
 /*
- * Tiles of the stacked form (gemm_stile, three gemm_tile records each):
- * C blocks with the same Q and the same pair sequence on the Y side (Y
- * panel, its leading dimension, k and factor at every position) form a
- * stack; its rows are the members' rows in 32-row tiles (a member of P
- * rows takes ceil(P / 32)), four row tiles to a tile.  Items (a stack's
- * tiles at one Q offset) are dealt to the XCD lists longest first, as in
- * the grouped order, so the workgroups an XCD runs together read the same
- * Y fragments.
+ * Tiles of the stacked forms (records of a head and R row tiles: R = 4,
+ * gemm_stile, three gemm_tile records; R = 8, gemm_stile8, five): C blocks
+ * with the same Q and the same pair sequence on the Y side (Y panel, its
+ * leading dimension, k and factor at every position) form a group, which
+ * is one stack (merge) or one stack per C block; a stack's rows are its
+ * C blocks' rows in 32-row tiles (a C block of P rows takes ceil(P / 32)),
+ * R row tiles to a tile.  Items (a group's tiles at one Q offset) are
+ * dealt to the XCD lists longest first, as in the grouped order, so the
+ * workgroups an XCD runs together read the same Y fragments.
  */
+template <int R>
 static void
-order_tiles_stacked(const contract_plan *pl, std::vector<gemm_tile> &out)
+order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
+    std::vector<gemm_tile> &out)
 {
-	const uint32_t BQ = 128;
-	std::map<std::vector<uint64_t>, uint32_t> sid;
-	std::vector<std::vector<uint32_t>> stacks;
+	/* groups: C blocks with the same Q and Y-side pair sequence */
+	std::map<std::vector<uint64_t>, uint32_t> gid;
+	std::vector<std::vector<uint32_t>> groups;
 	for (uint32_t i = 0; i < pl->cblocks.size(); i++) {
 		const gemm_cblock &cb = pl->cblocks[i];
 		std::vector<uint64_t> key = {cb.q, cb.pair_end - cb.pair_begin};
@@ -1480,31 +1483,40 @@ order_tiles_stacked(const contract_plan *pl, std::vector<gemm_tile> &out)
 		}
 		if (cb.pair_begin == cb.pair_end)
 			key.push_back(0x8000000000000000ull | i);	/* alone */
-		auto it = sid.find(key);
-		if (it == sid.end()) {
-			it = sid.emplace(key, (uint32_t)stacks.size()).first;
-			stacks.emplace_back();
+		auto it = gid.find(key);
+		if (it == gid.end()) {
+			it = gid.emplace(key, (uint32_t)groups.size()).first;
+			groups.emplace_back();
 		}
-		stacks[it->second].push_back(i);
+		groups[it->second].push_back(i);
 	}
+	/* a group's stacks (all its C blocks, or one each without merge) and
+	 * their rows in 32-row tiles */
+	std::vector<std::vector<std::vector<std::pair<uint32_t, uint32_t>>>>
+	    rows(groups.size());
 	struct item {
 		double work;
-		uint32_t s, q0;
+		uint32_t g, q0;
 	};
 	std::vector<item> items;
-	std::vector<std::vector<std::pair<uint32_t, uint32_t>>> rows(
-	    stacks.size());
-	for (uint32_t s = 0; s < stacks.size(); s++) {
-		for (uint32_t i : stacks[s])
+	for (uint32_t g = 0; g < groups.size(); g++) {
+		size_t nt = 0;
+		for (uint32_t i : groups[g]) {
+			if (!merge || rows[g].empty())
+				rows[g].emplace_back();
 			for (uint32_t p0 = 0; p0 < pl->cblocks[i].p; p0 += 32)
-				rows[s].push_back({i, p0});
-		const gemm_cblock &c0 = pl->cblocks[stacks[s][0]];
+				rows[g].back().push_back({i, p0});
+			if (!merge)
+				nt += (rows[g].back().size() + R - 1) / R;
+		}
+		if (merge)
+			nt = (rows[g].back().size() + R - 1) / R;
+		const gemm_cblock &c0 = pl->cblocks[groups[g][0]];
 		double kt = 1;
 		for (uint32_t p = c0.pair_begin; p < c0.pair_end; p++)
 			kt += pl->pairs[p].k;
-		const double nt = (double)((rows[s].size() + 3) / 4);
 		for (uint32_t q0 = 0; q0 < c0.q; q0 += BQ)
-			items.push_back({kt * nt, s, q0});
+			items.push_back({kt * (double)nt, g, q0});
 	}
 	std::vector<size_t> order(items.size());
 	for (size_t i = 0; i < order.size(); i++)
@@ -1512,40 +1524,49 @@ order_tiles_stacked(const contract_plan *pl, std::vector<gemm_tile> &out)
 	std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
 		return items[a].work > items[b].work;
 	});
+	/* a tile record: head, then R C-block indices and R first rows */
+	struct rec {
+		gemm_tile head;
+		uint32_t cb[R], p0[R];
+	};
+	static_assert(sizeof(rec) % sizeof(gemm_tile) == 0, "tile records");
 	double load[NXCD] = {0};
-	std::vector<gemm_stile> bins[NXCD];
+	std::vector<rec> bins[NXCD];
 	for (size_t o : order) {
 		const int x = (int)(std::min_element(load, load + NXCD) - load);
 		const item &it = items[o];
 		load[x] += it.work;
-		const auto &rw = rows[it.s];
-		for (size_t r0 = 0; r0 < rw.size(); r0 += 4) {
-			gemm_stile t;
-			t.head = {stacks[it.s][0], 0, it.q0, 0};
-			for (size_t r = 0; r < 4; r++) {
-				t.cb[r] = r0 + r < rw.size() ? rw[r0 + r].first :
-				    TILE_ROWS_NONE;
-				t.p0[r] = r0 + r < rw.size() ? rw[r0 + r].second : 0;
+		for (const auto &rw : rows[it.g])
+			for (size_t r0 = 0; r0 < rw.size(); r0 += R) {
+				rec t;
+				t.head = {rw[0].first, 0, it.q0, 0};
+				for (size_t r = 0; r < R; r++) {
+					const bool in = r0 + r < rw.size();
+					t.cb[r] = in ? rw[r0 + r].first :
+					    TILE_ROWS_NONE;
+					t.p0[r] = in ? rw[r0 + r].second : 0;
+				}
+				bins[x].push_back(t);
 			}
-			bins[x].push_back(t);
-		}
 	}
 	size_t per = 0;
 	for (auto &b : bins)
 		per = std::max(per, b.size());
-	static_assert(sizeof(gemm_stile) == 3 * sizeof(gemm_tile),
-	    "a stacked tile is three tile records");
-	gemm_stile none;
+	rec none;
 	memset(&none, 0, sizeof none);
 	none.head.cblock = 0xffffffffu;
-	std::vector<gemm_stile> flat(per * NXCD, none);
+	std::vector<rec> flat(per * NXCD, none);
 	for (int x = 0; x < NXCD; x++)
 		for (size_t s = 0; s < bins[x].size(); s++)
 			flat[s * NXCD + x] = bins[x][s];
-	out.resize(flat.size() * 3);
+	out.resize(flat.size() * (sizeof(rec) / sizeof(gemm_tile)));
 	if (!flat.empty())
-		memcpy(out.data(), flat.data(), flat.size() * sizeof(gemm_stile));
+		memcpy(out.data(), flat.data(), flat.size() * sizeof(rec));
 }
+static_assert(sizeof(gemm_stile) == 3 * sizeof(gemm_tile),
+    "a stacked FP64 tile is three tile records");
+static_assert(sizeof(gemm_stile8) == 5 * sizeof(gemm_tile),
+    "a stacked int8 tile is five tile records");
 
 /*
  * K split of FP64 tiles (not with XM_GEMM_KSPLIT=0 or the FMA kernel).
@@ -1751,6 +1772,20 @@ mfma_dw_eligible(const contract_plan *pl)
 	return true;
 }
 
+/*
+ * The int8 form's tiles hold one C block each unless XM_CRT_STACK=1
+ * merges a group's C blocks into one stack.  On the o=64 v=384 ladder the
+ * merged stacks stage 8% fewer fragments but ran 9.75 ms against 9.50
+ * (a group's C blocks in their own tiles, run together); the FP64 form,
+ * whose 128-row tiles pad the ladder's C blocks far more, gains from it.
+ */
+static bool
+stacked_crt(void)
+{
+	const char *e = getenv("XM_CRT_STACK");
+	return e && *e == '1';
+}
+
 void
 build_tiles(contract_plan *pl)
 {
@@ -1760,15 +1795,24 @@ build_tiles(contract_plan *pl)
 	std::vector<std::pair<uint32_t, uint32_t>> cand;
 	tile_candidates(pl, cand);
 	pl->ncand = (int)std::min<size_t>(cand.size(), MAX_TILE_CANDIDATES);
+	/* the int8 kernel reads stacked tile records only (one C block a
+	 * stack with XM_CRT_STACK=0) */
 	for (int c = 0; c < pl->ncand; c++)
 		pl->cand_stacked[c] = pl->type == XM_SCALAR_DOUBLE &&
-		    pl->precision == XM_PRECISION_NATIVE && cand[c].first == 128;
+		    ((pl->precision == XM_PRECISION_NATIVE &&
+		    cand[c].first == 128) ||
+		    pl->precision == XM_PRECISION_SPLIT_INT8);
 	mark_ksplit(pl);
 	for (int c = 0; c < pl->ncand; c++) {
 		pl->cand_bp[c] = cand[c].first;
 		pl->cand_bq[c] = cand[c].second;
 		if (pl->cand_stacked[c]) {
-			order_tiles_stacked(pl, pl->cand_tiles[c]);
+			if (pl->precision == XM_PRECISION_SPLIT_INT8)
+				order_tiles_stacked<8>(pl, cand[c].second,
+				    stacked_crt(), pl->cand_tiles[c]);
+			else
+				order_tiles_stacked<4>(pl, cand[c].second, true,
+				    pl->cand_tiles[c]);
 			pl->cand_fix[c].clear();
 		} else if (grouped_order(pl))
 			order_tiles_grouped(pl, cand[c].first, cand[c].second,

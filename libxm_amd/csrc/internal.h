@@ -312,6 +312,12 @@ struct gemm_stile {
 	gemm_tile head;
 	uint32_t cb[4], p0[4];
 };
+/* The same for the int8 form's 256 x 256 tiles: eight row tiles, five
+ * gemm_tile records. */
+struct gemm_stile8 {
+	gemm_tile head;
+	uint32_t cb[8], p0[8];
+};
 #define TILE_ROWS_NONE 0xffffffffu
 
 /* Scalars of one contraction call, read by the kernels from device

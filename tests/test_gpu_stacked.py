@@ -7,14 +7,15 @@ K split off the form's results are bit-identical to the 64 x 128 DMA
 kernel's, and they match the oracle to 1e-13.  The ladders below stack
 C blocks of 144 to 400 rows (partial 32-row tiles, stacks of 2 and 4); the
 C-panel case writes permuted C panels; alpha == 0 and beta == 0 keep
-libxm's rules exactly.
+libxm's rules exactly.  The int8 form's tiles can stack a group's C blocks
+the same way (XM_CRT_STACK=1): exact integer residues, so the same bits.
 """
 import os
 
 import numpy as np
 import pytest
 
-from libxm_amd import workloads
+from libxm_amd import workloads, xm
 from libxm_amd.spec import Spec, rel_err
 
 from helpers import build_filled
@@ -25,7 +26,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def env():
-    keys = ("XM_GEMM_KSPLIT", "XM_GEMM_TILE")
+    keys = ("XM_GEMM_KSPLIT", "XM_GEMM_TILE", "XM_CRT_STACK")
     old = {k: os.environ.get(k) for k in keys}
     yield os.environ
     for k, v in old.items():
@@ -131,3 +132,26 @@ def test_stacked_c_panels(gpulib, oracle, env):
         assert st["cout_bytes"] > 0, st
         want = run_contract(oracle, specs, data, al, be, idx)
         assert rel_err(got, want) <= 1e-13, (al, be)
+
+
+def test_int8_stacks_identical(gpulib, env):
+    """The int8 form with a group's C blocks stacked in shared tiles
+    (XM_CRT_STACK=1) against one C block a tile: the residue products are
+    exact integers, so the results are the same bits."""
+    w = workloads.ladder_d2h(64, 128, 2)
+    out = {}
+    for flag in ("0", "1"):
+        env["XM_CRT_STACK"] = flag
+        alloc = gpulib.allocator()
+        x = [s.build(gpulib, alloc) for s in w.specs]
+        _fill(gpulib, w, x, 17)
+        gpulib.set_precision(xm.PRECISION_SPLIT_INT8, None, 0)
+        gpulib.contract(0.5, x[0], x[1], 1.0, x[2], *w.idx)
+        st = gpulib.stats()
+        assert st["precision"] == xm.PRECISION_SPLIT_INT8, st["precision"]
+        out[flag] = w.c.read(gpulib, x[2])
+        for z in x:
+            gpulib.tensor_free(z)
+        gpulib.allocator_destroy(alloc)
+    for k in out["0"]:
+        assert np.array_equal(out["0"][k], out["1"][k]), k
