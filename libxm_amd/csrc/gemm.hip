@@ -1786,6 +1786,15 @@ stacked_crt(void)
 	return e && *e == '1';
 }
 
+/* The BF16 split form's 128 x 128 tiles stack a group's C blocks like the
+ * FP64 form (XM_SPLIT_STACK=0: one C block a stack). */
+static bool
+stacked_split(void)
+{
+	const char *e = getenv("XM_SPLIT_STACK");
+	return !(e && *e == '0');
+}
+
 void
 build_tiles(contract_plan *pl)
 {
@@ -1801,7 +1810,9 @@ build_tiles(contract_plan *pl)
 		pl->cand_stacked[c] = pl->type == XM_SCALAR_DOUBLE &&
 		    ((pl->precision == XM_PRECISION_NATIVE &&
 		    cand[c].first == 128) ||
-		    pl->precision == XM_PRECISION_SPLIT_INT8);
+		    pl->precision == XM_PRECISION_SPLIT_INT8 ||
+		    (pl->precision == XM_PRECISION_SPLIT_BF16 &&
+		    cand[c].first == 128));
 	mark_ksplit(pl);
 	for (int c = 0; c < pl->ncand; c++) {
 		pl->cand_bp[c] = cand[c].first;
@@ -1811,8 +1822,9 @@ build_tiles(contract_plan *pl)
 				order_tiles_stacked<8>(pl, cand[c].second,
 				    stacked_crt(), pl->cand_tiles[c]);
 			else
-				order_tiles_stacked<4>(pl, cand[c].second, true,
-				    pl->cand_tiles[c]);
+				order_tiles_stacked<4>(pl, cand[c].second,
+				    pl->precision == XM_PRECISION_NATIVE ||
+				    stacked_split(), pl->cand_tiles[c]);
 			pl->cand_fix[c].clear();
 		} else if (grouped_order(pl))
 			order_tiles_grouped(pl, cand[c].first, cand[c].second,

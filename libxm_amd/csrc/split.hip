@@ -205,7 +205,13 @@ ld_frag(const uint16_t *p)
 /*
  * Workgroup tile 128 (P) x 128 (Q), 8 waves, one workgroup per CU (two
  * waves per SIMD).  Wave w owns P rows (w & 3) * 32 + [0,32) and Q columns
- * (w >> 2) * 64 + [0,64) as two 32 x 32 MFMA tiles.  For every 16-wide k
+ * (w >> 2) * 64 + [0,64) as two 32 x 32 MFMA tiles.  A tile record
+ * (gemm_stile) names the C block and first row of each 32-row tile: the
+ * C blocks of a group that read the same Y slices with the same factors
+ * stack their rows (XM_SPLIT_STACK=0: one C block a tile), so the ladder's
+ * C blocks of 144-400 rows are not padded to multiples of 128 (its GEMM
+ * went from 15.0 to 13.5 ms at S = 5); an X fragment or exponent row comes
+ * from its row tile's C block's pair.  For every 16-wide k
  * step the workgroup stages the S slices of its 4 X and 4 Y row tiles (8S
  * fragments of 1 KiB) and the exponents of the step's chunk in LDS.  A
  * fragment is a contiguous 1 KiB run, so it is one wave-wide
@@ -256,7 +262,7 @@ sp_nstages(void)
 
 template <int S>
 __global__ void __launch_bounds__(SP_WG)
-k_gemm_split2(const gemm_tile *__restrict__ tiles,
+k_gemm_split2(const gemm_stile *__restrict__ tiles,
     const gemm_cblock *__restrict__ cbs, const split_pair *__restrict__ pairs,
     const gemm_scalars *__restrict__ scp)
 {
@@ -266,19 +272,46 @@ k_gemm_split2(const gemm_tile *__restrict__ tiles,
 	constexpr int WAIT = ((NST - 2) * (S + 1)) | (0x7 << 4) | (0xf << 8);
 	__shared__ sp_stage<S> st[NST];
 
-	const gemm_tile tl = tiles[blockIdx.x];
+	const gemm_stile *tp = tiles + blockIdx.x;
+	const gemm_tile tl = tp->head;
 	if (tl.cblock == 0xffffffffu)
 		return;
+	/* the first C block of the tile's stack: its pairs give Y, k steps
+	 * and factors, shared by every C block of the stack */
 	const gemm_cblock cb = cbs[tl.cblock];
 	const gemm_scalars sc = *scp;
 	const int tid = threadIdx.x, lane = tid & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const int wp = wave & 3, wq = wave >> 2;
-	const uint32_t pw = tl.p0 + wp * 32, qw = tl.q0 + wq * 64;
+	const uint32_t qw = tl.q0 + wq * 64;
 	const uint32_t pend = cb.pair_end;
 	const uint32_t pbeg = (sc.flags & 1) ? pend : cb.pair_begin;
-	const uint32_t prow = pw + (lane & 31);
-	const uint32_t rx0 = tl.p0 / 32, ry0 = tl.q0 / 32;
+	const uint32_t ry0 = tl.q0 / 32;
+	/* row tile r of the tile: rows rp0[r] .. + 31 of C block tp->cb[r]
+	 * (none: TILE_ROWS_NONE), whose pairs start at rb[r] (the first
+	 * C block's for none, so loads stay in range) */
+	uint32_t rb[4], rp0[4];
+	bool rhas[4];
+#pragma unroll
+	for (int r = 0; r < 4; r++) {
+		const uint32_t c = tp->cb[r];
+		rhas[r] = c != TILE_ROWS_NONE;
+		rb[r] = cbs[rhas[r] ? c : tl.cblock].pair_begin;
+		rp0[r] = rhas[r] ? tp->p0[r] : 0;
+	}
+	/* this wave's rows: row tile wp (selects, not indexed loads: an array
+	 * indexed at run time would go to scratch memory) */
+	uint32_t wb = rb[0], wr0 = rp0[0];
+	bool whas = rhas[0];
+#pragma unroll
+	for (int r = 1; r < 4; r++) {
+		wb = wp == r ? rb[r] : wb;
+		wr0 = wp == r ? rp0[r] : wr0;
+		whas = wp == r ? rhas[r] : whas;
+	}
+	const uint32_t mcb = tp->cb[wp];
+	const uint32_t prow = wr0 + (lane & 31);	/* row of C block mcb */
+	const uint32_t wrows = pairs[wb].xrows_pad;
 
 	f16v a2[2] = {}, a3[2] = {}, a4[2] = {}, a5[2] = {};
 	double acc[2][16];
@@ -308,27 +341,66 @@ k_gemm_split2(const gemm_tile *__restrict__ tiles,
 	uint32_t erows;
 	const int quarter = wave & 3;
 	const bool eisx = quarter < 2;
-	auto setup = [&](const split_pair px) {
+	/* pair ip of the stack (px: the first C block's): Y from px, X row
+	 * tile r from C block tp->cb[r]'s pair ip */
+	auto setup = [&](const split_pair px, uint32_t ip) {
+		const uint32_t ri = ip - pbeg;
 #pragma unroll
 		for (int i = 0; i < S; i++) {
 			const int f = wave + 8 * i;
 			const bool isx = f < FX;
 			const int g = isx ? f : f - FX;
-			const uint32_t rt = (isx ? rx0 : ry0) + g / S;
 			const int t = g % S;
 			dsrc[i] = sp_zero_page;
 			dstp[i] = 0;
-			if (rt * 32 < (isx ? px.xrows_pad : px.yrows_pad)) {
-				dsrc[i] = (const uint4 *)(uintptr_t)(isx ? px.xs :
-				    px.ys) + ((size_t)rt * px.nks * S + t) * 64;
-				dstp[i] = S * 64;
+			if (isx) {
+				const int r = g / S;
+				uint32_t b = rb[0], p0 = rp0[0];
+				bool has = rhas[0];
+#pragma unroll
+				for (int q = 1; q < 4; q++) {
+					b = r == q ? rb[q] : b;
+					p0 = r == q ? rp0[q] : p0;
+					has = r == q ? rhas[q] : has;
+				}
+				if (has) {
+					const split_pair xp = pairs[b + ri];
+					dsrc[i] = (const uint4 *)(uintptr_t)xp.xs +
+					    ((size_t)(p0 / 32) * xp.nks * S + t) * 64;
+					dstp[i] = S * 64;
+				}
+			} else {
+				const uint32_t rt = ry0 + g / S;
+				if (rt * 32 < px.yrows_pad) {
+					dsrc[i] = (const uint4 *)(uintptr_t)px.ys +
+					    ((size_t)rt * px.nks * S + t) * 64;
+					dstp[i] = S * 64;
+				}
 			}
 		}
-		erows = eisx ? px.xrows_pad : px.yrows_pad;
-		uint32_t row = (eisx ? tl.p0 : tl.q0) + 64 * (quarter & 1) + lane;
-		if (row >= erows)
-			row = 0;
-		esrc = (const int *)(uintptr_t)(eisx ? px.xe : px.ye) + row;
+		if (eisx) {
+			/* lanes 0-31: row tile 2 (quarter & 1), 32-63 the next */
+			const int r0 = 2 * (quarter & 1);
+			const split_pair e0 = pairs[(r0 == 0 ? rb[0] : rb[2]) + ri];
+			const split_pair e1 = pairs[(r0 == 0 ? rb[1] : rb[3]) + ri];
+			const bool h0 = r0 == 0 ? rhas[0] : rhas[2];
+			const bool h1 = r0 == 0 ? rhas[1] : rhas[3];
+			const uint32_t p00 = r0 == 0 ? rp0[0] : rp0[2];
+			const uint32_t p01 = r0 == 0 ? rp0[1] : rp0[3];
+			const bool hi = lane >= 32;
+			erows = hi ? e1.xrows_pad : e0.xrows_pad;
+			const uint64_t xe = hi ? e1.xe : e0.xe;
+			uint32_t row = (hi ? p01 : p00) + (lane & 31);
+			if (!(hi ? h1 : h0) || row >= erows)
+				row = 0;
+			esrc = (const int *)(uintptr_t)xe + row;
+		} else {
+			erows = px.yrows_pad;
+			uint32_t row = tl.q0 + 64 * (quarter & 1) + lane;
+			if (row >= erows)
+				row = 0;
+			esrc = (const int *)(uintptr_t)px.ye + row;
+		}
 	};
 	auto fetch = [&](sp_stage<S> *__restrict__ nx, uint32_t kx) {
 #pragma unroll
@@ -348,7 +420,7 @@ k_gemm_split2(const gemm_tile *__restrict__ tiles,
 	auto flush = [&](const sp_stage<S> *__restrict__ cur,
 	    const split_pair px) {
 		const double al = __dmul_rn(sc.alpha_re, px.a_re);
-		const int ep = prow < px.xrows_pad ?
+		const int ep = whas && prow < wrows ?
 		    cur->ex[wp * 32 + (lane & 31)] : 0;
 #pragma unroll
 		for (int j = 0; j < 2; j++)
@@ -400,7 +472,7 @@ k_gemm_split2(const gemm_tile *__restrict__ tiles,
 			k = 0;
 			if (++i < pend) {
 				p = pairs[i];
-				setup(p);
+				setup(p, i);
 			}
 		}
 	};
@@ -410,7 +482,7 @@ k_gemm_split2(const gemm_tile *__restrict__ tiles,
 	if (ip < pend) {
 		pr = pairs[ip];
 		fpr = pr;
-		setup(fpr);
+		setup(fpr, ip);
 	}
 #pragma unroll
 	for (int s = 0; s < NST - 1; s++)
@@ -479,7 +551,10 @@ k_gemm_split2(const gemm_tile *__restrict__ tiles,
 		step(&st[cs], &st[ns]);
 		cs = cs + 1 < NST ? cs + 1 : 0;
 	}
-	double *C = (double *)(uintptr_t)cb.c;
+	if (!whas)
+		return;
+	const gemm_cblock mc = cbs[mcb];
+	double *C = (double *)(uintptr_t)mc.c;
 	const bool bz = sc.flags & 2, none = pbeg == pend;
 #pragma unroll
 	for (int j = 0; j < 2; j++)
@@ -487,8 +562,8 @@ k_gemm_split2(const gemm_tile *__restrict__ tiles,
 		for (int i = 0; i < 16; i++) {
 			const uint32_t q = qw + 32 * j + 8 * (i / 4) +
 			    4 * (lane >> 5) + (i % 4);
-			if (prow < cb.p && q < cb.q) {
-				const size_t o = (size_t)prow + (size_t)q * cb.ldc;
+			if (prow < mc.p && q < mc.q) {
+				const size_t o = (size_t)prow + (size_t)q * mc.ldc;
 				double v;
 				if (bz)
 					v = none ? 0.0 : acc[j][i];
@@ -764,12 +839,14 @@ split_gemm(contract_plan *pl, hipStream_t s)
 		SDISPATCH(k_gemm_split, ss->nslices, g, dim3(SPLIT_WG), 0, s)(t,
 		    c, ss->d_pairs, sc);
 	} else {
-		const dim3 b(SP_WG);
+		/* stacked tile records, three gemm_tile each */
+		const dim3 b(SP_WG), g3((unsigned)(nt / 3));
+		const gemm_stile *ts = (const gemm_stile *)t;
 		switch (ss->nslices) {
-		case 3: k_gemm_split2<3><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
-		case 4: k_gemm_split2<4><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
-		case 5: k_gemm_split2<5><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
-		default: k_gemm_split2<6><<<g, b, 0, s>>>(t, c, ss->d_pairs, sc); break;
+		case 3: k_gemm_split2<3><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
+		case 4: k_gemm_split2<4><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
+		case 5: k_gemm_split2<5><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
+		default: k_gemm_split2<6><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
 		}
 	}
 	hipError_t e = hipGetLastError();

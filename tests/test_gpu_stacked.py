@@ -7,8 +7,9 @@ K split off the form's results are bit-identical to the 64 x 128 DMA
 kernel's, and they match the oracle to 1e-13.  The ladders below stack
 C blocks of 144 to 400 rows (partial 32-row tiles, stacks of 2 and 4); the
 C-panel case writes permuted C panels; alpha == 0 and beta == 0 keep
-libxm's rules exactly.  The int8 form's tiles can stack a group's C blocks
-the same way (XM_CRT_STACK=1): exact integer residues, so the same bits.
+libxm's rules exactly.  The BF16 split form's 128 x 128 tiles stack the
+same way (XM_SPLIT_STACK=0 keeps one C block a tile) and the int8 form's
+can (XM_CRT_STACK=1); each output's sums are unchanged, so the same bits.
 """
 import os
 
@@ -26,7 +27,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture
 def env():
-    keys = ("XM_GEMM_KSPLIT", "XM_GEMM_TILE", "XM_CRT_STACK")
+    keys = ("XM_GEMM_KSPLIT", "XM_GEMM_TILE", "XM_CRT_STACK",
+            "XM_SPLIT_STACK")
     old = {k: os.environ.get(k) for k in keys}
     yield os.environ
     for k, v in old.items():
@@ -155,3 +157,42 @@ def test_int8_stacks_identical(gpulib, env):
         gpulib.allocator_destroy(alloc)
     for k in out["0"]:
         assert np.array_equal(out["0"][k], out["1"][k]), k
+
+
+@pytest.mark.parametrize("nslices", [5, 6])
+def test_split_stacks_identical(gpulib, oracle, env, nslices):
+    """The BF16 split form with stacked tiles against one C block a tile:
+    the same bits, within 1e-12 of the oracle; alpha == 0 keeps beta * C
+    exactly."""
+    w = workloads.ladder_d2h(64, 128, 2)
+    out, data = {}, None
+    for flag in ("1", "0"):
+        env["XM_SPLIT_STACK"] = flag
+        alloc = gpulib.allocator()
+        x = [s.build(gpulib, alloc) for s in w.specs]
+        _fill(gpulib, w, x, 23)
+        if data is None:
+            data = [s.read(gpulib, z) for s, z in zip(w.specs, x)]
+        gpulib.set_precision(xm.PRECISION_SPLIT_BF16, nslices)
+        gpulib.contract(0.5, x[0], x[1], 1.0, x[2], *w.idx)
+        assert gpulib.stats()["gemm_kernel"] == "k_gemm_split2"
+        out[flag] = w.c.read(gpulib, x[2])
+        if flag == "1":
+            before = w.c.read(gpulib, x[2])
+            gpulib.contract(0.0, x[0], x[1], 2.0, x[2], *w.idx)
+            got = w.c.read(gpulib, x[2])
+            for k in before:
+                assert np.array_equal(got[k], 2.0 * before[k]), k
+        for z in x:
+            gpulib.tensor_free(z)
+        gpulib.allocator_destroy(alloc)
+    for k in out["0"]:
+        assert np.array_equal(out["0"][k], out["1"][k]), k
+    oal = oracle.allocator()
+    ot = [build_filled(oracle, oal, s, d) for s, d in zip(w.specs, data)]
+    oracle.contract(0.5, ot[0], ot[1], 1.0, ot[2], *w.idx)
+    want = w.c.read(oracle, ot[2])
+    for z in ot:
+        oracle.tensor_free(z)
+    oracle.allocator_destroy(oal)
+    assert rel_err(out["1"], want) <= 1e-12
