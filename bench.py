@@ -364,6 +364,9 @@ def ring_mode(lib, args):
              "gemm_ms": gemm_ms, "gemm_tflops": F / gemm_ms / 1e9,
              "block_pairs": st["npairs"], "c_blocks": st["nblocks_c"],
              "tile": [st["tile_p"], st["tile_q"]],
+             "gemm_kernel": st["gemm_kernel"],
+             "dense_classes": st["classes"],
+             "class_c_blocks": st["class_cblocks"],
              "first_call_ms_incl_plan": first,
              "unfold_panels": st["nunfold"],
              "unfold": {"bytes": unf, "ms": prep_ms,
@@ -386,9 +389,11 @@ def ring_mode(lib, args):
                        "frac_of_8tbs": cb / set_ms / 1e6 / HBM_PEAK}
         if name == "f64" and not args.no_cpu:
             f["cpu_baseline"] = cpu_ring_sample(lib, w, alloc, t, v, c, c2)
-        log("ring %s: %.3f ms/call %.2f TFLOP/s (GEMM %.3f ms), unfold %.1f GB "
-            "%.3f ms, fold %.1f GB %.3f ms, add %.1f GB/s, set %.1f GB/s" % (
-                name, ms, F / ms / 1e9, gemm_ms, unf / 1e9, prep_ms,
+        log("ring %s: %.3f ms/call %.2f TFLOP/s (GEMM %.3f ms, %s, %d "
+            "classes), unfold %.1f GB %.3f ms, fold %.1f GB %.3f ms, add "
+            "%.1f GB/s, set %.1f GB/s" % (
+                name, ms, F / ms / 1e9, gemm_ms, st["gemm_kernel"],
+                st["classes"], unf / 1e9, prep_ms,
                 st["cout_bytes"] / 1e9, st["post_ms"], f["xm_add"]["gbs"],
                 f["xm_set"]["gbs"]))
         out["forms"][name] = f

@@ -120,6 +120,10 @@ typedef struct {
 	/* bytes of blocks in an allocator's file read (pread into pinned
 	 * buffers, then staged like host blocks) and written back per call */
 	double file_bytes;
+	/* dense classes: C blocks whose terms form one GEMM of stacked
+	 * blocks (rows R x columns S over the same contracted blocks) run as
+	 * one C block each; the count of classes and of C blocks in them */
+	size_t classes, class_cblocks;
 } xm_contract_stats_t;
 
 void xm_contract_get_stats(xm_contract_stats_t *stats);

@@ -580,6 +580,262 @@ image_terms(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 	}
 }
 
+/* One GEMM term of a C block: block fa of a, block fb of b, merged factor
+ * f, contracted extent k. */
+struct cterm {
+	size_t fa, fb;
+	xm_scalar_t f;
+	uint64_t k;
+};
+
+/* A canonical C block of a plan: extents m (from a) and n (from b),
+ * orientation (swap: P = n and X comes from b), whether it is computed in
+ * a C panel, and its terms in libxm's order. */
+struct cinfo {
+	size_t fc;
+	uint64_t m, n;
+	bool swap, cpanel;
+	std::vector<cterm> terms;
+};
+
+/*
+ * Dense classes.  C blocks (r, s) whose terms read X blocks X(r, t) and
+ * Y blocks Y(t, s), t = 0 .. n-1, with the same extents k_t and one
+ * factor, for every r of a row set R and s of a column set S, are one GEMM
+ * C(R, S) = X(R, T) Y(T, S) of stacked blocks.  (The ring term
+ * c(i,j,a,b) = sum t(i,k,a,c) w(k,b,c,j) over tensors that store every
+ * symmetry-allowed block is eight such GEMMs, one per irrep of (i,a).)
+ * A class runs as one C block with one pair: its X blocks are permuted
+ * into one P-contiguous panel of sum_R P rows at their row and column
+ * offsets, its Y blocks likewise, C is computed in one panel folded back
+ * to its blocks.  Each block's K columns are padded with zeros to a
+ * multiple of 4: every 4-k group the GEMM kernels hand to an MFMA then
+ * lies within one term, as when each term is a pair of its own, so every
+ * output sums the same products in the same groups and order, and the
+ * results are the same bits (a zero's sign aside).  A class is formed
+ * when the per-C-block form permutes all of its operand blocks and
+ * computes its C blocks in panels too (so it adds no permute traffic) and
+ * its padded MFMA work (rows in 32s, columns in 128s) is at least 8% below
+ * the per-C-block form's (rows in 32s, columns in 64s, each pair's k in
+ * 16s).  XM_CLASS_GEMM=0 forms none, =1 every complete one.
+ */
+struct dense_class {
+	std::vector<uint32_t> members;		/* cinfo indices */
+	std::vector<uint32_t> rows, cols;	/* a member of each row, column */
+	std::vector<uint32_t> mrow, mcol;	/* each member's row, column */
+};
+
+static int
+class_mode(void)
+{
+	const char *e = getenv("XM_CLASS_GEMM");
+	if (e && *e == '0')
+		return 0;
+	return e && *e == '1' ? 2 : 1;
+}
+
+static uint64_t
+round_up(uint64_t x, uint64_t m)
+{
+	return (x + m - 1) / m * m;
+}
+
+/* Whether the per-C-block form reads block `flat` of t in place as an
+ * operand with free dims F and contracted dims Kd. */
+static bool
+read_in_place(const contract_plan *pl, const xm_tensor_t *t, size_t flat,
+    const xm_dim_t &F, const xm_dim_t &Kd)
+{
+	uint64_t ls[XM_MAX_DIM];
+	xm_dim_t d;
+	uint8_t mode;
+	uint32_t ld;
+	logical_strides(t, flat, ls, &d);
+	return direct_layout(ls, d, F, Kd, &mode, &ld) &&
+	    !(pcontig_panels(pl) && mode == PANEL_KCONTIG);
+}
+
+static std::vector<dense_class>
+find_classes(const xm_tensor_t *a, const xm_tensor_t *b, const masks_t &mk,
+    const contract_plan *pl, const std::vector<cinfo> &ci)
+{
+	std::vector<dense_class> out;
+	const int mode = class_mode();
+	const size_t N = ci.size();
+	if (mode == 0 || N == 0)
+		return out;
+	/* each C block's X and Y sequences (with swap and the k's) */
+	std::map<std::vector<uint64_t>, int64_t> xid, yid;
+	std::vector<int64_t> cx(N, -1), cy(N, -1);
+	for (size_t i = 0; i < N; i++) {
+		const cinfo &q = ci[i];
+		if (q.terms.empty())
+			continue;
+		bool uniform = true;
+		std::vector<uint64_t> kx = {q.swap}, ky = {q.swap};
+		for (const cterm &t : q.terms) {
+			uniform = uniform && t.f == q.terms[0].f;
+			kx.push_back(q.swap ? t.fb : t.fa);
+			kx.push_back(t.k);
+			ky.push_back(q.swap ? t.fa : t.fb);
+			ky.push_back(t.k);
+		}
+		if (!uniform)
+			continue;
+		cx[i] = xid.emplace(kx, (int64_t)xid.size()).first->second;
+		cy[i] = yid.emplace(ky, (int64_t)yid.size()).first->second;
+	}
+	/* components of the graph joining C blocks that share an X or a Y
+	 * sequence (union-find) */
+	std::vector<uint32_t> par(N);
+	for (size_t i = 0; i < N; i++)
+		par[i] = (uint32_t)i;
+	auto root = [&](uint32_t i) {
+		while (par[i] != i)
+			i = par[i] = par[par[i]];
+		return i;
+	};
+	std::vector<int64_t> fx(xid.size(), -1), fy(yid.size(), -1);
+	for (size_t i = 0; i < N; i++) {
+		if (cx[i] < 0)
+			continue;
+		for (int side = 0; side < 2; side++) {
+			int64_t &f = side ? fy[cy[i]] : fx[cx[i]];
+			if (f < 0)
+				f = (int64_t)i;
+			else
+				par[root((uint32_t)i)] = root((uint32_t)f);
+		}
+	}
+	std::map<uint32_t, size_t> cid;
+	std::vector<std::vector<uint32_t>> comps;
+	for (size_t i = 0; i < N; i++) {
+		if (cx[i] < 0)
+			continue;
+		auto it = cid.emplace(root((uint32_t)i), comps.size()).first;
+		if (it->second == comps.size())
+			comps.emplace_back();
+		comps[it->second].push_back((uint32_t)i);
+	}
+	const size_t esz = elem_size(pl->type);
+	for (auto &mem : comps) {
+		dense_class dc;
+		std::map<int64_t, uint32_t> rmap, smap;
+		std::set<std::pair<uint32_t, uint32_t>> seen;
+		bool ok = true;
+		for (uint32_t m : mem) {
+			auto r = rmap.emplace(cx[m], (uint32_t)rmap.size());
+			if (r.second)
+				dc.rows.push_back(m);
+			auto s = smap.emplace(cy[m], (uint32_t)smap.size());
+			if (s.second)
+				dc.cols.push_back(m);
+			dc.mrow.push_back(r.first->second);
+			dc.mcol.push_back(s.first->second);
+			ok = ok && seen.insert({r.first->second,
+			    s.first->second}).second &&
+			    ci[m].terms[0].f == ci[mem[0]].terms[0].f;
+		}
+		if (!ok || mem.size() != dc.rows.size() * dc.cols.size())
+			continue;
+		dc.members = mem;
+		if (mode == 1) {
+			/* padded MFMA work per C block against the class's, and
+			 * the bytes the class permutes beyond the per-block
+			 * form */
+			const cinfo &q0 = ci[mem[0]];
+			const bool sw = q0.swap;
+			const xm_tensor_t *xt = sw ? b : a, *yt = sw ? a : b;
+			const xm_dim_t &XF = sw ? mk.aidxb : mk.aidxa;
+			const xm_dim_t &XK = sw ? mk.cidxb : mk.cidxa;
+			const xm_dim_t &YF = sw ? mk.aidxa : mk.aidxb;
+			const xm_dim_t &YK = sw ? mk.cidxa : mk.cidxb;
+			uint64_t K4 = 0, K16 = 0, P = 0, Q = 0;
+			for (const cterm &t : q0.terms) {
+				K4 += round_up(t.k, 4);
+				K16 += round_up(t.k, 16);
+			}
+			double eb = 0, extra = 0;
+			for (uint32_t m : mem) {
+				const cinfo &q = ci[m];
+				const uint64_t p = sw ? q.n : q.m, qq = sw ? q.m : q.n;
+				eb += 2.0 * round_up(p, 32) * round_up(qq, 64) * K16;
+				if (!q.cpanel)
+					extra += 4.0 * p * qq * esz;
+			}
+			for (uint32_t r : dc.rows) {
+				const cinfo &q = ci[r];
+				P += sw ? q.n : q.m;
+				for (const cterm &t : q.terms)
+					if (read_in_place(pl, xt, sw ? t.fb : t.fa,
+					    XF, XK))
+						extra += 2.0 * (sw ? q.n : q.m) *
+						    t.k * esz;
+			}
+			for (uint32_t s : dc.cols) {
+				const cinfo &q = ci[s];
+				Q += sw ? q.m : q.n;
+				for (const cterm &t : q.terms)
+					if (read_in_place(pl, yt, sw ? t.fa : t.fb,
+					    YF, YK))
+						extra += 2.0 * (sw ? q.m : q.n) *
+						    t.k * esz;
+			}
+			const double ec = 2.0 * round_up(P, 32) *
+			    round_up(Q, 128) * K4;
+			if (extra > 0 || ec > 0.92 * eb)
+				continue;
+		}
+		out.push_back(std::move(dc));
+	}
+	return out;
+}
+
+/* The permute job writing block `flat` of t at dst as a P-contiguous
+ * (P x K) panel: free dims F, then contracted dims Kd. */
+static elem_job
+panel_job(const xm_tensor_t *t, size_t flat, const xm_dim_t &F,
+    const xm_dim_t &Kd, uint64_t dst)
+{
+	uint64_t ls[XM_MAX_DIM];
+	xm_dim_t d;
+	logical_strides(t, flat, ls, &d);
+	const xm_block &bl = t->blocks[flat];
+	elem_job j;
+	memset(&j, 0, sizeof j);
+	j.dst = dst;
+	j.src = bl.type == XM_BLOCK_TYPE_DERIVATIVE ?
+	    t->blocks[bl.data].data : bl.data;
+	j.n = xm_dim_dot_mask(&d, &F) * xm_dim_dot_mask(&d, &Kd);
+	for (size_t k = 0; k < F.n; k++) {
+		j.dims[j.ndim] = (uint32_t)d.i[F.i[k]];
+		j.sstride[j.ndim++] = ls[F.i[k]];
+	}
+	for (size_t k = 0; k < Kd.n; k++) {
+		j.dims[j.ndim] = (uint32_t)d.i[Kd.i[k]];
+		j.sstride[j.ndim++] = ls[Kd.i[k]];
+	}
+	return j;
+}
+
+/* A job writing zeros to a rows x cols window of a panel with leading
+ * dimension ld. */
+static elem_job
+zero_job(uint64_t dst, uint64_t rows, uint64_t cols, uint64_t ld)
+{
+	elem_job j;
+	memset(&j, 0, sizeof j);
+	j.dst = dst;
+	j.src_zero = 1;
+	j.n = rows * cols;
+	j.ndim = 2;
+	j.dims[0] = (uint32_t)rows;
+	j.dims[1] = (uint32_t)cols;
+	j.drows = rows;
+	j.dld = ld;
+	return j;
+}
+
 static contract_plan *
 build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
     const masks_t &mk, int precision, size_t lo = 0, size_t hi = SIZE_MAX,
@@ -609,52 +865,120 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 			pl->even_extents = false;
 	}
 
+	/* the C blocks: orientation and terms */
+	std::vector<cinfo> ci;
 	std::vector<block_pair> all;
 	std::vector<std::vector<uint32_t>> members;
 	for (size_t ic = lo; ic < hi; ic++) {
-		const xm_dim_t blkc = list[ic];
-		size_t fc = block_flat(c, blkc);
-		xm_dim_t dc = block_dims_flat(c, fc);
-		uint64_t m = xm_dim_dot_mask(&dc, &mk.cidxc);
-		uint64_t n = xm_dim_dot_mask(&dc, &mk.aidxc);
+		cinfo q;
+		q.fc = block_flat(c, list[ic]);
+		xm_dim_t dc = block_dims_flat(c, q.fc);
+		q.m = xm_dim_dot_mask(&dc, &mk.cidxc);
+		q.n = xm_dim_dot_mask(&dc, &mk.aidxc);
 
 		/* orientation of the C block */
 		uint64_t lsc[XM_MAX_DIM];
 		xm_dim_t dd;
-		logical_strides(c, fc, lsc, &dd);
+		logical_strides(c, q.fc, lsc, &dd);
 		uint64_t e;
-		bool swap;	/* P = n (X = B), Q = m (Y = A) */
-		bool cpanel = false;
+		q.cpanel = false;	/* swap: P = n (X = B), Q = m (Y = A) */
 		if (contiguous_run(lsc, dc, mk.cidxc, 1, &e) &&
-		    contiguous_run(lsc, dc, mk.aidxc, std::max<uint64_t>(m, 1), &e))
-			swap = false;
+		    contiguous_run(lsc, dc, mk.aidxc,
+		    std::max<uint64_t>(q.m, 1), &e))
+			q.swap = false;
 		else if (contiguous_run(lsc, dc, mk.aidxc, 1, &e) &&
-		    contiguous_run(lsc, dc, mk.cidxc, std::max<uint64_t>(n, 1), &e))
-			swap = true;
+		    contiguous_run(lsc, dc, mk.cidxc,
+		    std::max<uint64_t>(q.n, 1), &e))
+			q.swap = true;
 		else {
-			swap = false;
-			cpanel = true;
+			q.swap = false;
+			q.cpanel = true;
 		}
-		if (stage && arena_is_host(c->alloc, c->blocks[fc].data)) {
+		if (stage && arena_is_host(c->alloc, c->blocks[q.fc].data)) {
 			/* a host C block is computed in an HBM panel */
-			swap = false;
-			cpanel = true;
+			q.swap = false;
+			q.cpanel = true;
 		}
+		if (images != nullptr) {
+			ci.push_back(std::move(q));
+			continue;
+		}
+		size_t npre = enumerate_pairs(a, b, list[ic], mk, all, &members);
+		pl->npairs_pre += npre;
+		for (size_t i = 0; i < all.size(); i++) {
+			if (!all[i].nonzero)
+				continue;
+			xm_dim_t da = block_dims_flat(a, all[i].fa);
+			uint64_t k = xm_dim_dot_mask(&da, &mk.cidxa);
+			pl->flops_pre += 2.0 * mult * q.m * q.n * k;
+			if (all[i].alpha == xm_scalar_t(0))
+				continue;
+			pl->flops += 2.0 * mult * q.m * q.n * k;
+			pl->npairs++;
+			pl->pair_list.push_back({q.fc, all[i].fa, all[i].fb,
+			    all[i].alpha, q.m, q.n, k});
 
-		gemm_cblock cb;
-		memset(&cb, 0, sizeof cb);
-		cb.split_pair = KSPLIT_NONE;
-		cb.p = (uint32_t)(swap ? n : m);
-		cb.q = (uint32_t)(swap ? m : n);
-		cb.ldc = std::max<uint32_t>(cb.p, 1);
-		cb.pair_begin = (uint32_t)pl->pairs.size();
-		uint64_t cptr = c->blocks[fc].data;
-		if (cpanel) {
-			/* C panel: rows = cidxc dims, cols = aidxc dims */
-			uint64_t off = pl->ws_bytes;
-			pl->ws_bytes += (m * n * pb.esz + 255) / 256 * 256;
-			elem_job in, out;
-			memset(&in, 0, sizeof in);
+			/* pick the cheapest equivalent member of the merge
+			 * group (fewest operands needing a permute) */
+			block_pair best = all[i];
+			int best_cost = 3;
+			std::vector<uint32_t> cand = {(uint32_t)i};
+			for (uint32_t r : members[i])
+				cand.push_back(r);
+			for (uint32_t r : cand) {
+				if (r != i && !substitutable(a, b, mk, all[i], all[r]))
+					continue;
+				uint64_t ls[XM_MAX_DIM];
+				xm_dim_t d;
+				uint8_t md;
+				uint32_t ld;
+				int cost = 0;
+				logical_strides(a, all[r].fa, ls, &d);
+				cost += !direct_layout(ls, d, mk.aidxa, mk.cidxa, &md, &ld);
+				logical_strides(b, all[r].fb, ls, &d);
+				cost += !direct_layout(ls, d, mk.aidxb, mk.cidxb, &md, &ld);
+				if (cost < best_cost) {
+					best_cost = cost;
+					best = all[r];
+				}
+			}
+			if (best.fa != all[i].fa || best.fb != all[i].fb)
+				pl->nsubstituted++;
+			q.terms.push_back({best.fa, best.fb, all[i].alpha, k});
+		}
+		ci.push_back(std::move(q));
+	}
+	free(list);
+
+	/* dense classes (native precision, blocks in HBM) */
+	std::vector<dense_class> classes;
+	if (images == nullptr && !stage && precision == XM_PRECISION_NATIVE)
+		classes = find_classes(a, b, mk, pl, ci);
+	std::vector<int64_t> class_of(ci.size(), -1);
+	for (size_t k = 0; k < classes.size(); k++)
+		for (uint32_t m : classes[k].members)
+			class_of[m] = (int64_t)k;
+
+	const size_t esz = pb.esz;
+	auto ws_take = [&](uint64_t bytes) {
+		uint64_t off = pl->ws_bytes;
+		pl->ws_bytes += (bytes + 255) / 256 * 256;
+		return off;
+	};
+	/* C panel jobs of C block q: in (C block -> panel at off, pitch ld)
+	 * and out (panel -> C block) */
+	auto cpanel_jobs = [&](const cinfo &q, uint64_t off, uint64_t ld) {
+		xm_dim_t dc = block_dims_flat(c, q.fc);
+		uint64_t lsc[XM_MAX_DIM];
+		xm_dim_t dd;
+		logical_strides(c, q.fc, lsc, &dd);
+		const uint64_t cptr = c->blocks[q.fc].data;
+		elem_job in, out;
+		memset(&in, 0, sizeof in);
+		memset(&out, 0, sizeof out);
+		const uint64_t m = q.m, n = q.n;
+		if (q.cpanel) {
+			/* rows = cidxc dims, columns = aidxc dims */
 			in.dst = off;
 			in.src = cptr;
 			in.n = m * n;
@@ -666,8 +990,9 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 				in.dims[in.ndim] = (uint32_t)dc.i[mk.aidxc.i[k]];
 				in.sstride[in.ndim++] = lsc[mk.aidxc.i[k]];
 			}
+			in.drows = m;
+			in.dld = ld;
 			/* fold: C block element (canonical order) <- panel */
-			memset(&out, 0, sizeof out);
 			out.dst = cptr;
 			out.src = off;
 			out.n = m * n;
@@ -677,6 +1002,7 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 				pstride[mk.cidxc.i[k]] = s;
 				s *= dc.i[mk.cidxc.i[k]];
 			}
+			s = ld;
 			for (size_t k = 0; k < mk.aidxc.n; k++) {
 				pstride[mk.aidxc.i[k]] = s;
 				s *= dc.i[mk.aidxc.i[k]];
@@ -685,15 +1011,174 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 				out.dims[k] = (uint32_t)dc.i[k];
 				out.sstride[k] = pstride[k];
 			}
-			pl->cin_jobs.push_back(in);
-			pl->cout_jobs.push_back(out);
-			pl->unfold_bytes += 4.0 * m * n * pb.esz;
-			pl->cin_bytes += 2.0 * m * n * pb.esz;
-			pl->cout_bytes += 2.0 * m * n * pb.esz;
+		} else {
+			/* a C block laid out as its P x Q panel (a class
+			 * member): copied in and out of the class panel */
+			const uint64_t P = q.swap ? n : m, Q = q.swap ? m : n;
+			in.dst = off;
+			in.src = cptr;
+			in.n = P * Q;
+			in.ndim = 2;
+			in.dims[0] = (uint32_t)P;
+			in.dims[1] = (uint32_t)Q;
+			in.sstride[0] = 1;
+			in.sstride[1] = P;
+			in.drows = P;
+			in.dld = ld;
+			out.dst = cptr;
+			out.src = off;
+			out.n = P * Q;
+			out.ndim = 2;
+			out.dims[0] = (uint32_t)P;
+			out.dims[1] = (uint32_t)Q;
+			out.sstride[0] = 1;
+			out.sstride[1] = ld;
+		}
+		pl->cin_jobs.push_back(in);
+		pl->cout_jobs.push_back(out);
+		pl->unfold_bytes += 4.0 * m * n * esz;
+		pl->cin_bytes += 2.0 * m * n * esz;
+		pl->cout_bytes += 2.0 * m * n * esz;
+	};
+
+	/* one class: panels, permutes, one C block with one pair */
+	auto emit_class = [&](const dense_class &dc) {
+		const cinfo &q0 = ci[dc.members[0]];
+		const bool sw = q0.swap;
+		const xm_tensor_t *xt = sw ? b : a, *yt = sw ? a : b;
+		const xm_dim_t &XF = sw ? mk.aidxb : mk.aidxa;
+		const xm_dim_t &XK = sw ? mk.cidxb : mk.cidxa;
+		const xm_dim_t &YF = sw ? mk.aidxa : mk.aidxb;
+		const xm_dim_t &YK = sw ? mk.cidxa : mk.cidxb;
+		auto rows_of = [&](const cinfo &q) { return q.swap ? q.n : q.m; };
+		auto cols_of = [&](const cinfo &q) { return q.swap ? q.m : q.n; };
+		std::vector<uint64_t> roff, coff, koff;
+		uint64_t P = 0, Q = 0, K = 0;
+		for (uint32_t r : dc.rows) {
+			roff.push_back(P);
+			P += rows_of(ci[r]);
+		}
+		for (uint32_t s : dc.cols) {
+			coff.push_back(Q);
+			Q += cols_of(ci[s]);
+		}
+		for (const cterm &t : q0.terms) {
+			koff.push_back(K);
+			K += round_up(t.k, 4);
+		}
+		const uint64_t ldx = round_up(P, 4), ldy = round_up(Q, 4);
+		if (ldx > 0xffffffffull || ldy > 0xffffffffull ||
+		    K > 0xffffffffull)
+			XM_FATAL("xm_contract", "class panel too large");
+		const uint64_t ox = ws_take(ldx * K * esz);
+		const uint64_t oy = ws_take(ldy * K * esz);
+		const uint64_t oc = ws_take(ldx * Q * esz);
+		for (size_t r = 0; r < dc.rows.size(); r++) {
+			const cinfo &q = ci[dc.rows[r]];
+			for (size_t t = 0; t < q.terms.size(); t++) {
+				elem_job j = panel_job(xt, sw ? q.terms[t].fb :
+				    q.terms[t].fa, XF, XK,
+				    ox + (roff[r] + koff[t] * ldx) * esz);
+				j.drows = rows_of(q);
+				j.dld = ldx;
+				pl->unfold_jobs.push_back(j);
+				pl->unfold_src_ws.push_back(false);
+				pl->unfold_bytes += 2.0 * j.n * esz;
+				pl->panel_bytes += 2.0 * j.n * esz;
+			}
+		}
+		for (size_t s = 0; s < dc.cols.size(); s++) {
+			const cinfo &q = ci[dc.cols[s]];
+			for (size_t t = 0; t < q.terms.size(); t++) {
+				elem_job j = panel_job(yt, sw ? q.terms[t].fa :
+				    q.terms[t].fb, YF, YK,
+				    oy + (coff[s] + koff[t] * ldy) * esz);
+				j.drows = cols_of(q);
+				j.dld = ldy;
+				pl->unfold_jobs.push_back(j);
+				pl->unfold_src_ws.push_back(false);
+				pl->unfold_bytes += 2.0 * j.n * esz;
+				pl->panel_bytes += 2.0 * j.n * esz;
+			}
+		}
+		/* zeros: each term's K padding and the rows past P (Q) */
+		std::vector<elem_job> zj;
+		for (size_t t = 0; t < q0.terms.size(); t++) {
+			const uint64_t k = q0.terms[t].k, pad = round_up(k, 4) - k;
+			if (!pad)
+				continue;
+			zj.push_back(zero_job(ox + (koff[t] + k) * ldx * esz,
+			    ldx, pad, ldx));
+			zj.push_back(zero_job(oy + (koff[t] + k) * ldy * esz,
+			    ldy, pad, ldy));
+		}
+		if (ldx > P)
+			zj.push_back(zero_job(ox + P * esz, ldx - P, K, ldx));
+		if (ldy > Q)
+			zj.push_back(zero_job(oy + Q * esz, ldy - Q, K, ldy));
+		for (const elem_job &j : zj) {
+			pl->unfold_jobs.push_back(j);
+			pl->unfold_src_ws.push_back(false);
+			pl->unfold_bytes += 1.0 * j.n * esz;
+			pl->panel_bytes += 1.0 * j.n * esz;
+		}
+		for (size_t i = 0; i < dc.members.size(); i++)
+			cpanel_jobs(ci[dc.members[i]], oc + (roff[dc.mrow[i]] +
+			    coff[dc.mcol[i]] * ldx) * esz, ldx);
+
+		gemm_cblock cb;
+		memset(&cb, 0, sizeof cb);
+		cb.split_pair = KSPLIT_NONE;
+		cb.c = oc;
+		cb.p = (uint32_t)P;
+		cb.q = (uint32_t)Q;
+		cb.ldc = (uint32_t)ldx;
+		cb.pair_begin = (uint32_t)pl->pairs.size();
+		cb.pair_end = cb.pair_begin + 1;
+		cb.flags = 1;
+		gemm_pair gp;
+		memset(&gp, 0, sizeof gp);
+		gp.x = ox;
+		gp.y = oy;
+		gp.k = (uint32_t)K;
+		gp.xmode = gp.ymode = PANEL_PCONTIG;
+		gp.ldx = (uint32_t)ldx;
+		gp.ldy = (uint32_t)ldy;
+		gp.a_re = q0.terms[0].f.real();
+		gp.a_im = q0.terms[0].f.imag();
+		pl->pairs.push_back(gp);
+		pl->x_in_ws.push_back(true);
+		pl->y_in_ws.push_back(true);
+		pl->pair_swap.push_back(false);
+		pl->cblocks.push_back(cb);
+		pl->cpanel.push_back(true);
+		pl->nclasses++;
+		pl->class_cblocks += dc.members.size();
+	};
+
+	for (size_t i = 0; i < ci.size(); i++) {
+		const cinfo &q = ci[i];
+		if (class_of[i] >= 0) {
+			const dense_class &dc = classes[class_of[i]];
+			if (dc.members[0] == i)
+				emit_class(dc);
+			continue;
+		}
+		const bool swap = q.swap;
+		gemm_cblock cb;
+		memset(&cb, 0, sizeof cb);
+		cb.split_pair = KSPLIT_NONE;
+		cb.p = (uint32_t)(swap ? q.n : q.m);
+		cb.q = (uint32_t)(swap ? q.m : q.n);
+		cb.ldc = std::max<uint32_t>(cb.p, 1);
+		cb.pair_begin = (uint32_t)pl->pairs.size();
+		if (q.cpanel) {
+			uint64_t off = ws_take(q.m * q.n * esz);
+			cpanel_jobs(q, off, std::max<uint64_t>(q.m, 1));
 			cb.c = off;
 			cb.flags |= 1;
 		} else {
-			cb.c = cptr;
+			cb.c = c->blocks[q.fc].data;
 		}
 
 		/* one GEMM term of this C block: operands fa (free dims Fa,
@@ -727,61 +1212,16 @@ build_plan(const xm_tensor_t *a, const xm_tensor_t *b, const xm_tensor_t *c,
 			 * slicing stage) */
 			pl->pair_swap.push_back(swap);
 		};
-		if (images != nullptr) {
-			image_terms(a, b, c, images, fc, mk, pl, mult, m, n, emit);
-			cb.pair_end = (uint32_t)pl->pairs.size();
-			pl->cblocks.push_back(cb);
-			pl->cpanel.push_back(cpanel);
-			continue;
-		}
-		size_t npre = enumerate_pairs(a, b, blkc, mk, all, &members);
-		pl->npairs_pre += npre;
-		for (size_t i = 0; i < all.size(); i++) {
-			if (!all[i].nonzero)
-				continue;
-			xm_dim_t da = block_dims_flat(a, all[i].fa);
-			uint64_t k = xm_dim_dot_mask(&da, &mk.cidxa);
-			pl->flops_pre += 2.0 * mult * m * n * k;
-			if (all[i].alpha == xm_scalar_t(0))
-				continue;
-			pl->flops += 2.0 * mult * m * n * k;
-			pl->npairs++;
-			pl->pair_list.push_back({fc, all[i].fa, all[i].fb,
-			    all[i].alpha, m, n, k});
-
-			/* pick the cheapest equivalent member of the merge
-			 * group (fewest operands needing a permute) */
-			block_pair best = all[i];
-			int best_cost = 3;
-			std::vector<uint32_t> cand = {(uint32_t)i};
-			for (uint32_t r : members[i])
-				cand.push_back(r);
-			for (uint32_t r : cand) {
-				if (r != i && !substitutable(a, b, mk, all[i], all[r]))
-					continue;
-				uint64_t ls[XM_MAX_DIM];
-				xm_dim_t d;
-				uint8_t md;
-				uint32_t ld;
-				int cost = 0;
-				logical_strides(a, all[r].fa, ls, &d);
-				cost += !direct_layout(ls, d, mk.aidxa, mk.cidxa, &md, &ld);
-				logical_strides(b, all[r].fb, ls, &d);
-				cost += !direct_layout(ls, d, mk.aidxb, mk.cidxb, &md, &ld);
-				if (cost < best_cost) {
-					best_cost = cost;
-					best = all[r];
-				}
-			}
-			if (best.fa != all[i].fa || best.fb != all[i].fb)
-				pl->nsubstituted++;
-			emit(best.fa, best.fb, mk.aidxa, mk.aidxb, all[i].alpha, k);
-		}
+		if (images != nullptr)
+			image_terms(a, b, c, images, q.fc, mk, pl, mult, q.m, q.n,
+			    emit);
+		else
+			for (const cterm &t : q.terms)
+				emit(t.fa, t.fb, mk.aidxa, mk.aidxb, t.f, t.k);
 		cb.pair_end = (uint32_t)pl->pairs.size();
 		pl->cblocks.push_back(cb);
-		pl->cpanel.push_back(cpanel);
+		pl->cpanel.push_back(q.cpanel);
 	}
-	free(list);
 	pl->nunfold = pl->unfold_jobs.size();
 	if (!pb.host_blocks.empty()) {
 		/* merge host blocks of one slab into ranges, one staged copy
@@ -1557,6 +1997,8 @@ record_stats(contract_plan *pl, bool cached, xm_scalar_t alpha,
 		st.panel_bytes += q->panel_bytes;
 		st.cin_bytes += q->cin_bytes;
 		st.cout_bytes += q->cout_bytes;
+		st.classes += q->nclasses;
+		st.class_cblocks += q->class_cblocks;
 		st.permute_tiled += (ops ? q->unfold_l.ntiled : 0) +
 		    (cin ? q->cin_l.ntiled : 0) + q->cout_l.ntiled;
 		st.permute_gather += (ops ? q->unfold_l.ngather : 0) +

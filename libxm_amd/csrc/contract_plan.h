@@ -45,6 +45,8 @@ struct contract_plan {
 	size_t npairs = 0, npairs_pre = 0, nblocks_c = 0, nunfold = 0;
 	size_t nsubstituted = 0;
 	std::vector<plan_pair> pair_list;
+	/* dense classes run as one C block each, and the C blocks in them */
+	size_t nclasses = 0, class_cblocks = 0;
 
 	/* grouped GEMM descriptors (host copies) */
 	std::vector<gemm_cblock> cblocks;

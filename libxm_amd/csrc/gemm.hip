@@ -1708,9 +1708,11 @@ order_tiles(const contract_plan *pl, uint32_t BP, uint32_t BQ,
 			tiles[s * NXCD + x] = bins[x][s];
 }
 
-/* Every X and Y panel P-contiguous, rows and leading dimension even, base
- * 16-byte aligned: what k_gemm_f64_dma's 16-byte row-pair DMAs need.
- * XM_GEMM_DMA=0 turns the DMA kernel off. */
+/* Every X and Y panel P-contiguous with an even leading dimension that
+ * holds the C block's rows (columns) rounded up to even, base 16-byte
+ * aligned: what k_gemm_f64_dma's 16-byte row-pair DMAs need (a pair past
+ * an odd row count reads the padding row, whose products reach only
+ * outputs past the C block).  XM_GEMM_DMA=0 turns the DMA kernel off. */
 static bool
 f64_dma_eligible(const contract_plan *pl)
 {
@@ -1721,22 +1723,21 @@ f64_dma_eligible(const contract_plan *pl)
 	for (const gemm_cblock &cb : pl->cblocks) {
 		if (cb.pair_begin == cb.pair_end)
 			continue;
-		if (cb.p % 2 || cb.q % 2)
-			return false;
 		for (uint32_t i = cb.pair_begin; i < cb.pair_end; i++) {
 			const gemm_pair &g = pl->pairs[i];
 			if (g.xmode != PANEL_PCONTIG || g.ymode != PANEL_PCONTIG ||
-			    g.ldx % 2 || g.ldy % 2 || g.x % 16 || g.y % 16)
+			    g.ldx % 2 || g.ldy % 2 || g.x % 16 || g.y % 16 ||
+			    g.ldx < cb.p + cb.p % 2 || g.ldy < cb.q + cb.q % 2)
 				return false;
 		}
 	}
 	return true;
 }
 
-/* FP32 plans whose C blocks have P and Q divisible by 4 and whose panels
- * have a leading dimension divisible by 4 and 16-byte aligned bases: what
- * k_gemm_mfma_dw's 16-byte DMAs (4 rows of a column) need.  XM_GEMM_DMA=0
- * keeps the dword form. */
+/* FP32 plans whose panels have a leading dimension divisible by 4 that
+ * holds the C block's rows (columns) rounded up to 4, and 16-byte aligned
+ * bases: what k_gemm_mfma_dw's 16-byte DMAs (4 rows of a column) need.
+ * XM_GEMM_DMA=0 keeps the dword form. */
 static bool
 mfma_x4_eligible(const contract_plan *pl)
 {
@@ -1746,11 +1747,11 @@ mfma_x4_eligible(const contract_plan *pl)
 	for (const gemm_cblock &cb : pl->cblocks) {
 		if (cb.pair_begin == cb.pair_end)
 			continue;
-		if (cb.p % 4 || cb.q % 4)
-			return false;
 		for (uint32_t i = cb.pair_begin; i < cb.pair_end; i++) {
 			const gemm_pair &g = pl->pairs[i];
-			if (g.ldx % 4 || g.ldy % 4 || g.x % 16 || g.y % 16)
+			if (g.ldx % 4 || g.ldy % 4 || g.x % 16 || g.y % 16 ||
+			    g.ldx < (cb.p + 3) / 4 * 4 ||
+			    g.ldy < (cb.q + 3) / 4 * 4)
 				return false;
 		}
 	}

@@ -206,6 +206,10 @@ void launch_fill_random(const seg_desc *dsegs, size_t nseg, size_t nchunks,
  * destination block (canonical layout), the source element is at
  * src + sum_d e_d * sstride[d] (strides already include the source
  * block's permutation).  Operation codes are listed in kernels_elem.hip.
+ * A pitched destination (drows != 0) is a drows x (n / drows) matrix
+ * with leading dimension dld: element e goes to
+ * e % drows + (e / drows) * dld.  drows must be the product of the
+ * extents of some leading destination dims.
  */
 #define XM_ELEM_MAXDIM 8
 struct elem_job {
@@ -217,6 +221,7 @@ struct elem_job {
 	uint32_t dims[XM_ELEM_MAXDIM];	/* destination extents */
 	uint64_t sstride[XM_ELEM_MAXDIM];
 	double f_re, f_im;		/* per-job factor */
+	uint64_t drows, dld;		/* destination pitch (0: contiguous) */
 };
 
 enum elem_op {

@@ -238,6 +238,8 @@ struct elem_job_dev {
 	fastdiv div[XM_ELEM_MAXDIM];	/* merged destination extents */
 	uint64_t sstride[XM_ELEM_MAXDIM];	/* and their source strides */
 	double f_re, f_im;
+	fastdiv drow;			/* pitched destination: rows ... */
+	uint64_t dld;			/* ... and leading dimension (0: none) */
 };
 
 struct elem_tile_dev {
@@ -278,6 +280,17 @@ src_offset(const elem_job_dev &jb, uint32_t i)
 		i = q;
 	}
 	return off;
+}
+
+/* Destination offset of element i (a pitched destination's rows are
+ * runs of drow.d elements, dld apart). */
+__device__ __forceinline__ uint64_t
+dst_offset(const elem_job_dev &jb, uint32_t i)
+{
+	if (!jb.dld)
+		return i;
+	const uint32_t q = fdiv_q(i, jb.drow);
+	return (uint64_t)(i - q * jb.drow.d) + (uint64_t)q * jb.dld;
 }
 
 template <int OP> struct reads_dst {
@@ -334,10 +347,11 @@ k_elem(const elem_job_dev *jobs, uint32_t njobs, double al_re, double al_im,
 		S x = mk<S>(0.0, 0.0);
 		if (present)
 			x = src[src_offset(jb, (uint32_t)i)];
+		const uint64_t di = dst_offset(jb, (uint32_t)i);
 		D d = mk<D>(0.0, 0.0);
 		if (reads_dst<OP>::v)
-			d = dst[i];
-		dst[i] = elem_op<D, S, OP>(d, x, present, f, alpha, alpha_zero);
+			d = dst[di];
+		dst[di] = elem_op<D, S, OP>(d, x, present, f, alpha, alpha_zero);
 	}
 }
 
@@ -588,27 +602,23 @@ launch_fill_random(const seg_desc *dsegs, size_t nseg, size_t nchunks,
 }
 
 /*
- * Tiled form of a job (dims/str: merged destination extents and their
- * source strides), or false where the source's fastest dim is destination
- * dim 0 (the gather form reads runs then).  Near transposes (dims 0..t
- * within one 4096-element chunk) ran 4.2-4.3 TB/s gathered and 4.4-4.7
- * tiled on 4-D FP64 copies; far ones 0.8-2.6 gathered.
+ * Tiled form of a job (dims/str/dstr: merged destination extents, their
+ * source and destination strides), or false where the source's fastest
+ * dim is destination dim 0 (the gather form reads runs then) or
+ * destination dim 0 is not contiguous.  Near transposes (dims 0..t within
+ * one 4096-element chunk) ran 4.2-4.3 TB/s gathered and 4.4-4.7 tiled on
+ * 4-D FP64 copies; far ones 0.8-2.6 gathered.
  */
 static bool
 plan_tiled(elem_tile_dev &b, const uint32_t *dims, const uint64_t *str,
-    uint32_t nd, size_t esz)
+    const uint64_t *dstr, uint32_t nd, size_t esz)
 {
 	uint32_t t = 0;
 	for (uint32_t d = 1; d < nd; d++)
 		if (str[d] < str[t])
 			t = d;
-	if (t == 0)
+	if (t == 0 || dstr[0] != 1)
 		return false;
-	uint64_t dstr[XM_ELEM_MAXDIM], s = 1;
-	for (uint32_t d = 0; d < nd; d++) {
-		dstr[d] = s;
-		s *= dims[d];
-	}
 	b.A = dims[0];
 	b.B = dims[t];
 	b.nta = (b.A + TILE_MAX - 1) / TILE_MAX;
@@ -661,27 +671,40 @@ to_dev(const elem_job *jobs, size_t n, size_t esz, bool tiling,
 		const elem_job &a = jobs[i];
 		if (a.n > 0xffffffffull)
 			XM_FATAL("libxm", "block of more than 2^32 elements");
-		/* merge dimension d + 1 into d where the source walks them as
-		 * one run (stride[d+1] = stride[d] * dims[d]): an identity
-		 * permutation becomes one dimension, one multiply per element */
+		/* merge dimension d + 1 into d where the source and the
+		 * destination walk them as one run (stride[d+1] = stride[d] *
+		 * dims[d] on both sides): an identity permutation becomes one
+		 * dimension, one multiply per element.  dstr: destination
+		 * strides (a pitched destination's column dims step by dld) */
+		const bool pitched = a.drows != 0 && a.drows < a.n;
+		if (pitched && (a.drows > 0xffffffffull || a.dld < a.drows))
+			XM_FATAL("libxm", "bad destination pitch");
 		uint32_t dims[XM_ELEM_MAXDIM];
-		uint64_t str[XM_ELEM_MAXDIM];
+		uint64_t str[XM_ELEM_MAXDIM], dstr[XM_ELEM_MAXDIM];
 		uint32_t nd = 0;
+		uint64_t cum = 1;	/* destination elements before dim d */
 		for (uint32_t d = 0; d < a.ndim; d++) {
+			if (pitched && cum < a.drows && cum * a.dims[d] > a.drows)
+				XM_FATAL("libxm", "destination pitch splits a dim");
+			const uint64_t ds = !pitched || cum < a.drows ? cum :
+			    cum / a.drows * a.dld;
 			if (a.dims[d] == 1)
 				continue;	/* contributes offset 0 */
+			cum *= a.dims[d];
 			if (nd > 0 && str[nd - 1] * dims[nd - 1] == a.sstride[d] &&
+			    dstr[nd - 1] * dims[nd - 1] == ds &&
 			    (uint64_t)dims[nd - 1] * a.dims[d] <= 0xffffffffu) {
 				dims[nd - 1] *= a.dims[d];
 				continue;
 			}
 			dims[nd] = a.dims[d];
+			dstr[nd] = ds;
 			str[nd++] = a.sstride[d];
 		}
 		elem_tile_dev tb;
 		memset(&tb, 0, sizeof tb);
 		if (tiling && !a.src_zero && nd >= 2 &&
-		    plan_tiled(tb, dims, str, nd, esz)) {
+		    plan_tiled(tb, dims, str, dstr, nd, esz)) {
 			tb.dst = a.dst;
 			tb.src = a.src;
 			tb.f_re = a.f_re;
@@ -705,6 +728,8 @@ to_dev(const elem_job *jobs, size_t n, size_t esz, bool tiling,
 		}
 		b.f_re = a.f_re;
 		b.f_im = a.f_im;
+		b.drow = make_fdiv(pitched ? (uint32_t)a.drows : 1);
+		b.dld = pitched ? a.dld : 0;
 		b.first_chunk = cg;
 		cg += (a.n + CHUNK - 1) / CHUNK;
 		g.push_back(b);

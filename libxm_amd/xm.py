@@ -70,7 +70,8 @@ class ContractStats(C.Structure):
                 ("permute_gather", C.c_size_t),
                 ("staged_batches", C.c_int), ("staged_bytes", C.c_double),
                 ("ksplit_tiles", C.c_size_t), ("err_bound", C.c_double),
-                ("gemm_kernel", C.c_char * 32), ("file_bytes", C.c_double)]
+                ("gemm_kernel", C.c_char * 32), ("file_bytes", C.c_double),
+                ("classes", C.c_size_t), ("class_cblocks", C.c_size_t)]
 
 
 def dim(seq):

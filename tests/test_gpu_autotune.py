@@ -30,13 +30,20 @@ def _run(lib, w, t, v, c):
     return st, w.c.read(lib, c)
 
 
-@pytest.mark.parametrize("make,cands", [
-    (lambda: workloads.ladder_d2h(32, 96, 3), [(128, 128), (64, 128)]),
+@pytest.mark.parametrize("make,cands,classes", [
+    (lambda: workloads.ladder_d2h(32, 96, 3), [(128, 128), (64, 128)], ""),
     (lambda: workloads.ring_d2h(24, 72, 5, lo=4, hi=12),
-     [(64, 128), (64, 64)]),
-], ids=["ladder", "ring"])
-def test_tuning_sequence_and_identical_results(gpulib, make, cands):
+     [(64, 128), (64, 64)], "0"),
+    (lambda: workloads.ring_d2h(24, 72, 5, lo=4, hi=12),
+     [(128, 128), (64, 128)], "1"),
+], ids=["ladder", "ring", "ring-classes"])
+def test_tuning_sequence_and_identical_results(gpulib, make, cands, classes,
+                                               monkeypatch):
+    """The ring with each C block a GEMM of its own (odd extents: the
+    register-staged kernel) and as dense classes (padded panels: the DMA
+    kernels)."""
     os.environ.pop("XM_GEMM_TILE", None)
+    monkeypatch.setenv("XM_CLASS_GEMM", classes)
     w = make()
     alloc = gpulib.allocator()
     t, v, c = (s.build(gpulib, alloc) for s in w.specs)

@@ -56,9 +56,12 @@ def test_dma_forms_identical(gpulib, oracle, no_dma_env, t, tol, tile,
     run_forms(gpulib, oracle, w, t, tol, lambda dma: KERNEL[t, tile, dma])
 
 
-def test_x4_fallback(gpulib, oracle, no_dma_env):
+def test_x4_fallback(gpulib, oracle, no_dma_env, monkeypatch):
     """An FP32 ring with block extents of 4..10 (P or Q not divisible by
-    4): the dword form with DMA on or off, still matching the oracle."""
+    4), each C block a GEMM of its own (a dense class's panels have
+    leading dimensions of a multiple of 4): the dword form with DMA on or
+    off, still matching the oracle."""
+    monkeypatch.setenv("XM_CLASS_GEMM", "0")
     t = xm.SCALAR_FLOAT
     w = workloads.ring_d2h(20, 48, 3, type_=t, lo=4, hi=10)
     run_forms(gpulib, oracle, w, t, 1e-5, lambda dma: "k_gemm_mfma_dw")

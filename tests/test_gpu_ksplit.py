@@ -40,11 +40,13 @@ def _workload():
 def env():
     """The K split is a property of the 64 x 128 and 64 x 64 forms (the
     stacked 128 x 128 form, which plans the DMA kernel can run try first,
-    runs all of K): the tests pin 64 x 128 unless they choose a shape."""
+    runs all of K): the tests pin 64 x 128 unless they choose a shape, and
+    keep each C block a GEMM of its own (no dense classes)."""
     keys = ("XM_GEMM_KSPLIT", "XM_GEMM_TILE", "XM_GEMM_DMA",
-            "XM_GEMM_PERSIST")
+            "XM_GEMM_PERSIST", "XM_CLASS_GEMM")
     old = {k: os.environ.get(k) for k in keys}
     os.environ["XM_GEMM_TILE"] = "64x128"
+    os.environ["XM_CLASS_GEMM"] = "0"
     yield os.environ
     for k, v in old.items():
         if v is None:
