@@ -1,4 +1,6 @@
 """Helpers shared by the test modules."""
+import zlib
+
 import numpy as np
 
 from libxm_amd import xm
@@ -59,3 +61,31 @@ def max_abs_diff(x, y):
                           np.asarray(y[k], np.complex128))) if np.size(x[k]) else 0.0
         m = max(m, float(d))
     return m
+
+
+def case_seed(name):
+    """A seed fixed by a case's name (Python's str hash changes from
+    process to process)."""
+    return zlib.crc32(name.encode())
+
+
+def magnitude_err(got, want, specs, data, alpha, beta, idx):
+    """Frobenius norm of got - want over that of the magnitudes the result
+    is built from, |alpha| sum_k |a| |b| + |beta| |c|: the error measure of
+    the split forms' error bounds, which cancellation in the sums does not
+    inflate as it does the error relative to the result."""
+    ia, ib, ic = idx
+    A = np.abs(specs[0].dense(data[0]))
+    B = np.abs(specs[1].dense(data[1]))
+    C = np.abs(specs[2].dense(data[2]))
+    S = abs(alpha) * np.einsum("%s,%s->%s" % (ia, ib, ic), A, B) + \
+        abs(beta) * C
+    scale = specs[2].blocks_of(S)
+    num = den = 0.0
+    for k in want:
+        num += float(np.sum(np.abs(np.asarray(got[k], dtype=np.complex128) -
+                                   np.asarray(want[k])) ** 2))
+        den += float(np.sum(np.abs(scale[k]) ** 2))
+    if den == 0.0:
+        return 0.0 if num == 0.0 else float("inf")
+    return (num / den) ** 0.5
