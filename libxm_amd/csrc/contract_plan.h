@@ -77,7 +77,7 @@ struct contract_plan {
 	int tile_active = 0;		/* candidate of the last launch */
 	bool f64_dma = false;		/* k_gemm_f64_dma can run the 64 x 128 tiles */
 	bool mfma_dw = false;		/* k_gemm_mfma_dw can run the plan */
-	bool mfma_x4 = false;		/* ... with 16-byte DMAs (FP32) */
+	bool mfma_x4 = false;		/* ... with 16-byte DMAs */
 	bool even_extents = false;	/* every C block's P and Q even */
 	/* plans of xm_contract_noalloc: one tile shape (the first
 	 * candidate, never tuned) and no K split, whose partial sums would

@@ -1007,9 +1007,10 @@ lds_dma(const void *src, void *dst)
 		    (fd_lds_t *)dst, 4, 0, 0);
 }
 
-/* DW = 4 (FP32 plans whose panels and C blocks allow it): a lane's DMA
- * moves 4 consecutive rows of one column, a quarter of the DMAs (and of
- * their address arithmetic) of the dword form. */
+/* DW = 4 (plans whose panels allow it): a lane's DMA moves 16 bytes of
+ * one column, 4 consecutive FP32 rows, 2 complex64 rows or one complex128
+ * row, a quarter of the DMAs (and of their address arithmetic) of the
+ * dword form. */
 template <typename T, int BQ, int M32_BK, int DW = 1>
 __global__ void __launch_bounds__(GEMM_WG,
     sizeof(typename mdw<T>::R) == 8 ? 2 : 4)
@@ -1027,7 +1028,7 @@ k_gemm_mfma_dw(const gemm_tile *__restrict__ tiles,
 	constexpr int SX = M32_BK * XW, SY = M32_BK * YW;	/* of a stage */
 	static_assert(SX % (256 * DW) == 0 && SY % (256 * DW) == 0,
 	    "DMAs per wave");
-	static_assert(DW == 1 || W == 1, "16-byte DMAs for FP32 only");
+	static_assert(DW == 1 || 4 % W == 0, "16-byte DMAs: whole elements");
 	__shared__ __align__(16) float st[2][SX + SY];	/* [stage][X, Y][k][row] */
 
 	const gemm_tile tl = tiles[blockIdx.x];
@@ -1342,6 +1343,27 @@ xcd_lists(const contract_plan *pl, uint32_t BP, uint32_t BQ,
  */
 #define QCHUNK 1
 
+/*
+ * A group of one C block with at least 2 TILE_RECT tiles along P and along
+ * Q (a dense class: thousands of rows and columns) is cut into items of
+ * TILE_RECT x TILE_RECT tiles instead of columns: the tiles an XCD runs
+ * together then read TILE_RECT X and TILE_RECT Y panels per k step rather
+ * than every X panel of the C block.  TILE_RECT is 8 (XM_TILE_RECT sets
+ * it, 0 keeps columns).  On the o=96 v=480 ring ...
+ */
+static uint32_t
+tile_rect(void)
+{
+	static int v = -1;
+	if (v < 0) {
+		const char *e = getenv("XM_TILE_RECT");
+		v = e && *e ? atoi(e) : 8;
+		if (v < 0)
+			v = 0;
+	}
+	return (uint32_t)v;
+}
+
 static bool
 grouped_order(const contract_plan *pl)
 {
@@ -1353,6 +1375,7 @@ grouped_order(const contract_plan *pl)
 struct tile_item {
 	double work;
 	uint32_t g, q0, q1;
+	uint32_t p0, p1;	/* rows of a one-C-block group's item */
 };
 
 /* The groups, their items and the items of each XCD list (longest
@@ -1390,10 +1413,25 @@ group_items(const contract_plan *pl, uint32_t BP, uint32_t BQ,
 		double np = 0;
 		for (uint32_t i : groups[g])
 			np += (pl->cblocks[i].p + BP - 1) / BP;
+		const uint32_t TR = tile_rect();
+		if (TR && groups[g].size() == 1 && np >= 2 * TR &&
+		    c0.q >= 2 * TR * BQ) {
+			for (uint32_t p0 = 0; p0 < c0.p; p0 += TR * BP)
+				for (uint32_t q0 = 0; q0 < c0.q; q0 += TR * BQ) {
+					const uint32_t p1 = std::min(c0.p,
+					    p0 + TR * BP);
+					const uint32_t q1 = std::min(c0.q,
+					    q0 + TR * BQ);
+					items.push_back({kt * ((p1 - p0 + BP - 1) /
+					    BP) * ((q1 - q0 + BQ - 1) / BQ), g, q0,
+					    q1, p0, p1});
+				}
+			continue;
+		}
 		for (uint32_t q0 = 0; q0 < c0.q; q0 += QCHUNK * BQ) {
 			const uint32_t q1 = std::min(c0.q, q0 + QCHUNK * BQ);
 			items.push_back({kt * np * ((q1 - q0 + BQ - 1) / BQ), g, q0,
-			    q1});
+			    q1, 0, 0xffffffffu});
 		}
 	}
 	std::vector<size_t> order(items.size());
@@ -1430,7 +1468,8 @@ order_tiles_grouped(const contract_plan *pl, uint32_t BP, uint32_t BQ,
 			for (uint32_t q0 = it.q0; q0 < it.q1; q0 += BQ)
 				for (uint32_t i : groups[it.g]) {
 					const gemm_cblock &cb = pl->cblocks[i];
-					for (uint32_t p0 = 0; p0 < cb.p; p0 += BP) {
+					const uint32_t pe = std::min(cb.p, it.p1);
+					for (uint32_t p0 = it.p0; p0 < pe; p0 += BP) {
 						if (cb.split_pair == KSPLIT_NONE) {
 							bins[x].push_back({i, p0, q0, 0});
 							continue;
@@ -1490,33 +1529,59 @@ order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
 		}
 		groups[it->second].push_back(i);
 	}
-	/* a group's stacks (all its C blocks, or one each without merge) and
-	 * their rows in 32-row tiles */
-	std::vector<std::vector<std::vector<std::pair<uint32_t, uint32_t>>>>
-	    rows(groups.size());
+	/* a group's stacks (all its C blocks, or one each without merge),
+	 * their rows in 32-row tiles, R to a tile record */
+	struct rec {
+		gemm_tile head;
+		uint32_t cb[R], p0[R];
+	};
+	static_assert(sizeof(rec) % sizeof(gemm_tile) == 0, "tile records");
+	std::vector<std::vector<rec>> recs(groups.size());
 	struct item {
 		double work;
-		uint32_t g, q0;
+		uint32_t g, q0, q1, s0, s1;	/* Q range, record range */
 	};
 	std::vector<item> items;
 	for (uint32_t g = 0; g < groups.size(); g++) {
-		size_t nt = 0;
+		std::vector<std::vector<std::pair<uint32_t, uint32_t>>> rows;
 		for (uint32_t i : groups[g]) {
-			if (!merge || rows[g].empty())
-				rows[g].emplace_back();
+			if (!merge || rows.empty())
+				rows.emplace_back();
 			for (uint32_t p0 = 0; p0 < pl->cblocks[i].p; p0 += 32)
-				rows[g].back().push_back({i, p0});
-			if (!merge)
-				nt += (rows[g].back().size() + R - 1) / R;
+				rows.back().push_back({i, p0});
 		}
-		if (merge)
-			nt = (rows[g].back().size() + R - 1) / R;
+		for (const auto &rw : rows)
+			for (size_t r0 = 0; r0 < rw.size(); r0 += R) {
+				rec t;
+				t.head = {rw[0].first, 0, 0, 0};
+				for (size_t r = 0; r < R; r++) {
+					const bool in = r0 + r < rw.size();
+					t.cb[r] = in ? rw[r0 + r].first :
+					    TILE_ROWS_NONE;
+					t.p0[r] = in ? rw[r0 + r].second : 0;
+				}
+				recs[g].push_back(t);
+			}
 		const gemm_cblock &c0 = pl->cblocks[groups[g][0]];
 		double kt = 1;
 		for (uint32_t p = c0.pair_begin; p < c0.pair_end; p++)
 			kt += pl->pairs[p].k;
+		const uint32_t nt = (uint32_t)recs[g].size(), TR = tile_rect();
+		if (TR && nt >= 2 * TR && c0.q >= 2 * TR * BQ) {
+			for (uint32_t s0 = 0; s0 < nt; s0 += TR)
+				for (uint32_t q0 = 0; q0 < c0.q; q0 += TR * BQ) {
+					const uint32_t s1 = std::min(nt, s0 + TR);
+					const uint32_t q1 = std::min(c0.q,
+					    q0 + TR * BQ);
+					items.push_back({kt * (s1 - s0) *
+					    ((q1 - q0 + BQ - 1) / BQ), g, q0, q1, s0,
+					    s1});
+				}
+			continue;
+		}
 		for (uint32_t q0 = 0; q0 < c0.q; q0 += BQ)
-			items.push_back({kt * (double)nt, g, q0});
+			items.push_back({kt * (double)nt, g, q0,
+			    std::min(c0.q, q0 + BQ), 0, nt});
 	}
 	std::vector<size_t> order(items.size());
 	for (size_t i = 0; i < order.size(); i++)
@@ -1524,28 +1589,16 @@ order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
 	std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
 		return items[a].work > items[b].work;
 	});
-	/* a tile record: head, then R C-block indices and R first rows */
-	struct rec {
-		gemm_tile head;
-		uint32_t cb[R], p0[R];
-	};
-	static_assert(sizeof(rec) % sizeof(gemm_tile) == 0, "tile records");
 	double load[NXCD] = {0};
 	std::vector<rec> bins[NXCD];
 	for (size_t o : order) {
 		const int x = (int)(std::min_element(load, load + NXCD) - load);
 		const item &it = items[o];
 		load[x] += it.work;
-		for (const auto &rw : rows[it.g])
-			for (size_t r0 = 0; r0 < rw.size(); r0 += R) {
-				rec t;
-				t.head = {rw[0].first, 0, it.q0, 0};
-				for (size_t r = 0; r < R; r++) {
-					const bool in = r0 + r < rw.size();
-					t.cb[r] = in ? rw[r0 + r].first :
-					    TILE_ROWS_NONE;
-					t.p0[r] = in ? rw[r0 + r].second : 0;
-				}
+		for (uint32_t q0 = it.q0; q0 < it.q1; q0 += BQ)
+			for (uint32_t s = it.s0; s < it.s1; s++) {
+				rec t = recs[it.g][s];
+				t.head.q0 = q0;
 				bins[x].push_back(t);
 			}
 	}
@@ -1734,24 +1787,26 @@ f64_dma_eligible(const contract_plan *pl)
 	return true;
 }
 
-/* FP32 plans whose panels have a leading dimension divisible by 4 that
- * holds the C block's rows (columns) rounded up to 4, and 16-byte aligned
- * bases: what k_gemm_mfma_dw's 16-byte DMAs (4 rows of a column) need.
- * XM_GEMM_DMA=0 keeps the dword form. */
+/* Plans whose panels have 16-byte aligned bases and a leading dimension
+ * divisible by the elements of 16 bytes, E (4 FP32, 2 complex64, 1
+ * complex128), that holds the C block's rows (columns) rounded up to E:
+ * what k_gemm_mfma_dw's 16-byte DMAs need.  XM_GEMM_DMA=0 keeps the dword
+ * form. */
 static bool
 mfma_x4_eligible(const contract_plan *pl)
 {
 	const char *e = getenv("XM_GEMM_DMA");
-	if (!pl->mfma_dw || pl->type != XM_SCALAR_FLOAT || (e && *e == '0'))
+	if (!pl->mfma_dw || (e && *e == '0'))
 		return false;
+	const uint32_t E = 16 / (uint32_t)elem_size(pl->type);
 	for (const gemm_cblock &cb : pl->cblocks) {
 		if (cb.pair_begin == cb.pair_end)
 			continue;
 		for (uint32_t i = cb.pair_begin; i < cb.pair_end; i++) {
 			const gemm_pair &g = pl->pairs[i];
-			if (g.ldx % 4 || g.ldy % 4 || g.x % 16 || g.y % 16 ||
-			    g.ldx < (cb.p + 3) / 4 * 4 ||
-			    g.ldy < (cb.q + 3) / 4 * 4)
+			if (g.ldx % E || g.ldy % E || g.x % 16 || g.y % 16 ||
+			    g.ldx < (cb.p + E - 1) / E * E ||
+			    g.ldy < (cb.q + E - 1) / E * E)
 				return false;
 		}
 	}
@@ -1934,11 +1989,9 @@ gemm_kernel_name(const contract_plan *pl)
 			return persistent(pl) ? "k_gemm_f64_dma_q" :
 			    "k_gemm_f64_dma";
 		return "k_gemm_f64";
-	case XM_SCALAR_FLOAT:
+	default:
 		if (pl->mfma_x4)
 			return "k_gemm_mfma_dw<x4>";
-		return pl->mfma_dw ? "k_gemm_mfma_dw" : "k_gemm_simple";
-	default:
 		return pl->mfma_dw ? "k_gemm_mfma_dw" : "k_gemm_simple";
 	}
 }
@@ -2008,7 +2061,11 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_FLOAT_COMPLEX:
-		if (pl->mfma_dw)
+		if (pl->mfma_x4)
+			k_gemm_mfma_dw<cx<float>, 64, 16, 4><<<g, b, 0, s>>>(
+			    pl->d_tiles, pl->d_cblocks, pl->d_pairs,
+			    pl->d_scalars);
+		else if (pl->mfma_dw)
 			k_gemm_mfma_dw<cx<float>, 64, 16><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		else
@@ -2016,7 +2073,11 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		break;
 	case XM_SCALAR_DOUBLE_COMPLEX:
-		if (pl->mfma_dw)
+		if (pl->mfma_x4)
+			k_gemm_mfma_dw<cx<double>, 64, 8, 4><<<g, b, 0, s>>>(
+			    pl->d_tiles, pl->d_cblocks, pl->d_pairs,
+			    pl->d_scalars);
+		else if (pl->mfma_dw)
 			k_gemm_mfma_dw<cx<double>, 64, 8><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
 		else

@@ -1,9 +1,9 @@
 """The DMA-staged GEMM forms against their alternatives.
 
-FP32 plans whose C blocks and panels allow it stage operands with 16-byte
-DMAs (four rows of a column per lane) instead of dword DMAs; FP64 plans
-with even extents run the DMA-staged 64 x 128 kernel instead of the
-register-staged one.  Each pair runs the same MFMA sequence per output, so
+FP32 and complex plans whose C blocks and panels allow it stage operands
+with 16-byte DMAs (16 bytes of a column per lane) instead of dword DMAs;
+FP64 plans with even extents run the DMA-staged 64 x 128 kernel instead
+of the register-staged one.  Each pair runs the same MFMA sequence per output, so
 results must be bit-identical (XM_GEMM_DMA=0 selects the alternative),
 and match the oracle to the type's tolerance.  The kernel each call ran
 (stats gemm_kernel) is checked too, and a plan with extents that are not
@@ -54,6 +54,17 @@ def test_dma_forms_identical(gpulib, oracle, no_dma_env, t, tol, tile,
     monkeypatch.setenv("XM_GEMM_TILE", tile)
     w = workloads.ladder_d2h(32, 96, 11, type_=t)
     run_forms(gpulib, oracle, w, t, tol, lambda dma: KERNEL[t, tile, dma])
+
+
+@pytest.mark.parametrize("t,tol", [(xm.SCALAR_FLOAT_COMPLEX, 1e-5),
+                                   (xm.SCALAR_DOUBLE_COMPLEX, 1e-13)],
+                         ids=["c64", "c128"])
+def test_dma_forms_complex(gpulib, oracle, no_dma_env, t, tol):
+    """Complex plans: a 16-byte DMA moves two complex64 rows or one
+    complex128 row of a column (64 x 64 tiles)."""
+    w = workloads.ladder_d2h(32, 96, 11, type_=t)
+    run_forms(gpulib, oracle, w, t, tol, lambda dma: "k_gemm_mfma_dw<x4>"
+              if dma == "1" else "k_gemm_mfma_dw")
 
 
 def test_x4_fallback(gpulib, oracle, no_dma_env, monkeypatch):
