@@ -1349,7 +1349,11 @@ xcd_lists(const contract_plan *pl, uint32_t BP, uint32_t BQ,
  * TILE_RECT x TILE_RECT tiles instead of columns: the tiles an XCD runs
  * together then read TILE_RECT X and TILE_RECT Y panels per k step rather
  * than every X panel of the C block.  TILE_RECT is 8 (XM_TILE_RECT sets
- * it, 0 keeps columns).  On the o=96 v=480 ring ...
+ * it, 0 keeps columns).  On the o=96 v=480 ring (eight classes of about
+ * 5800 x 5800 x 5800) the stacked FP64 GEMM ran 46.7 ms against 47.7 with
+ * columns (rectangles of 4 and 6: 46.8, of 16: 47.8); its MFMA pipe was
+ * busy 85% of the time already with columns, so the HBM reads they cost
+ * (106 GB a launch, 2.2 TB/s) were mostly hidden.
  */
 static uint32_t
 tile_rect(void)
