@@ -1397,7 +1397,10 @@ plan_upload(contract_plan *pl)
 		pl->d_cand_tiles[c] = (gemm_tile *)(base + offs[3 + c]);
 		pl->d_cand_fix[c] = (gemm_tile *)(base +
 		    offs[3 + pl->ncand + c]);
-		nslot = std::max(nslot, pl->cand_fix[c].size());
+		/* a split stacked tile (three records) takes two slots a
+		 * half */
+		nslot = std::max(nslot, pl->cand_stacked[c] ?
+		    pl->cand_fix[c].size() / 3 * 2 : pl->cand_fix[c].size());
 	}
 	/* counters of the persistent GEMM's tile queues, beside the K-split
 	 * sums */

@@ -604,6 +604,48 @@ k_gemm_f64_dma_q(const gemm_tile *__restrict__ tiles, uint32_t per,
  */
 #define FS_BP 128
 #define FS_WG 512
+/* doubles of one half's partial sums of a split stacked tile */
+#define FS_KSLOT (2 * KSPLIT_SLOT)
+static_assert(FS_BP * FD_BQ <= FS_KSLOT, "a stacked tile fits its slot");
+
+/* C = beta * C + (half 0 + half 1) over one split stacked tile per
+ * workgroup (fix[] holds the first halves); row tile r of the tile holds
+ * rows p0[r] .. of C block cb[r]. */
+__global__ void __launch_bounds__(GEMM_WG)
+k_ksplit_sum_s(const gemm_stile *__restrict__ fix,
+    const gemm_cblock *__restrict__ cbs, const double *__restrict__ ksw,
+    const gemm_scalars *__restrict__ scp)
+{
+	const gemm_stile tl = fix[blockIdx.x];
+	const gemm_scalars sc = *scp;
+	const double *W0 = ksw + (size_t)(tl.head.ksplit - 1) * FS_KSLOT;
+	const double *W1 = W0 + FS_KSLOT;
+	const bool bz = sc.flags & 2, none = sc.flags & 1;
+	for (int e = threadIdx.x; e < FS_BP * FD_BQ; e += GEMM_WG) {
+		const int pt = e % FS_BP, r = pt / 32;
+		const uint32_t m = r == 0 ? tl.cb[0] : r == 1 ? tl.cb[1] :
+		    r == 2 ? tl.cb[2] : tl.cb[3];
+		if (m == TILE_ROWS_NONE)
+			continue;
+		const uint32_t p0 = r == 0 ? tl.p0[0] : r == 1 ? tl.p0[1] :
+		    r == 2 ? tl.p0[2] : tl.p0[3];
+		const gemm_cblock cb = cbs[m];
+		const uint32_t p = p0 + pt % 32, q = tl.head.q0 + e / FS_BP;
+		if (p < cb.p && q < cb.q) {
+			double *C = (double *)(uintptr_t)cb.c;
+			const size_t o = (size_t)p + (size_t)q * cb.ldc;
+			double v;
+			if (bz)
+				v = none ? 0.0 : __dadd_rn(W0[e], W1[e]);
+			else if (none)
+				v = __dmul_rn(sc.beta_re, C[o]);
+			else
+				v = __dadd_rn(__dmul_rn(sc.beta_re, C[o]),
+				    __dadd_rn(W0[e], W1[e]));
+			C[o] = v;
+		}
+	}
+}
 
 template <int FD_BK>
 struct fs_stage {
@@ -615,7 +657,7 @@ template <int FD_BK, int FD_NST, int MINW>
 __global__ void __launch_bounds__(FS_WG, MINW)
 k_gemm_f64_dma_s(const gemm_stile *__restrict__ tiles,
     const gemm_cblock *__restrict__ cbs, const gemm_pair *__restrict__ pairs,
-    const gemm_scalars *__restrict__ scp)
+    const gemm_scalars *__restrict__ scp, double *__restrict__ ksw)
 {
 	constexpr int MT = 2, NT = 4;
 	constexpr int NXD = FD_BK, NYD = FD_BK;	/* one column per DMA */
@@ -727,15 +769,18 @@ k_gemm_f64_dma_s(const gemm_stile *__restrict__ tiles,
 
 	/* compute cursor (ip, kc, pr) with n steps left; fetch cursor
 	 * FD_NST - 1 steps ahead with fn left, its lane X in (fx, fldx) */
-	uint32_t ip = pbeg, kc = 0, n = pbeg == pend ? 0 : cb.steps;
-	uint32_t fip = ip, fk = 0, fn = n;
+	/* (a split tile runs one half of the steps; the stack's C blocks
+	 * have the same pair sequence, so the head's cut is everyone's) */
+	uint32_t ip, kc, n;
+	ksplit_steps(tl.head, cb, pbeg, ip, kc, n);
+	uint32_t fip = ip, fk = kc, fn = n;
 	gemm_pair pr, fpr;
 	uint64_t fx = 0;
 	uint32_t fldx = 0;
 	if (n) {
 		pr = pairs[ip];
 		fpr = pr;
-		xpanel(0, fx, fldx);
+		xpanel(ip - cb.pair_begin, fx, fldx);
 	}
 	auto fadvance = [&]() {
 		fk += FD_BK;
@@ -808,6 +853,21 @@ k_gemm_f64_dma_s(const gemm_stile *__restrict__ tiles,
 	/* this wave's rows: rows p0[wr] + wp' of C block mcb */
 	if (mcb == TILE_ROWS_NONE)
 		return;
+	if (tl.head.ksplit) {
+		/* the half's sums, element (p, q) of the tile at p + 128 q */
+		double *W = ksw + (size_t)(tl.head.ksplit - 1) * FS_KSLOT;
+#pragma unroll
+		for (int i = 0; i < MT; i++)
+#pragma unroll
+			for (int j = 0; j < NT; j++)
+#pragma unroll
+				for (int r = 0; r < 4; r++) {
+					const int p = wp + i * 16 + (lane & 15);
+					const int q = wq + j * 16 + 4 * r + (lane >> 4);
+					W[p + q * FS_BP] = acc[i][j][r];
+				}
+		return;
+	}
 	const gemm_cblock mc = cbs[mcb];
 	double *C = (double *)(uintptr_t)mc.c;
 	const bool bz = sc.flags & 2;
@@ -1507,13 +1567,31 @@ order_tiles_grouped(const contract_plan *pl, uint32_t BP, uint32_t BQ,
  * workgroups an XCD runs together read the same Y fragments.
  */
 template <int R>
+struct stacked_order {
+	struct rec {
+		gemm_tile head;
+		uint32_t cb[R], p0[R];
+	};
+	struct item {
+		double work;
+		uint32_t g, q0, q1, s0, s1;	/* Q range, record range */
+	};
+	std::vector<std::vector<uint32_t>> groups;
+	std::vector<std::vector<rec>> recs;	/* a group's records */
+	std::vector<item> items;
+	std::vector<size_t> lists[NXCD];	/* items per XCD, in order */
+};
+
+template <int R>
 static void
-order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
-    std::vector<gemm_tile> &out)
+stacked_items(const contract_plan *pl, uint32_t BQ, bool merge,
+    stacked_order<R> &so)
 {
+	typedef typename stacked_order<R>::rec rec;
+	static_assert(sizeof(rec) % sizeof(gemm_tile) == 0, "tile records");
 	/* groups: C blocks with the same Q and Y-side pair sequence */
 	std::map<std::vector<uint64_t>, uint32_t> gid;
-	std::vector<std::vector<uint32_t>> groups;
+	std::vector<std::vector<uint32_t>> &groups = so.groups;
 	for (uint32_t i = 0; i < pl->cblocks.size(); i++) {
 		const gemm_cblock &cb = pl->cblocks[i];
 		std::vector<uint64_t> key = {cb.q, cb.pair_end - cb.pair_begin};
@@ -1535,17 +1613,7 @@ order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
 	}
 	/* a group's stacks (all its C blocks, or one each without merge),
 	 * their rows in 32-row tiles, R to a tile record */
-	struct rec {
-		gemm_tile head;
-		uint32_t cb[R], p0[R];
-	};
-	static_assert(sizeof(rec) % sizeof(gemm_tile) == 0, "tile records");
-	std::vector<std::vector<rec>> recs(groups.size());
-	struct item {
-		double work;
-		uint32_t g, q0, q1, s0, s1;	/* Q range, record range */
-	};
-	std::vector<item> items;
+	so.recs.assign(groups.size(), {});
 	for (uint32_t g = 0; g < groups.size(); g++) {
 		std::vector<std::vector<std::pair<uint32_t, uint32_t>>> rows;
 		for (uint32_t i : groups[g]) {
@@ -1564,48 +1632,76 @@ order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
 					    TILE_ROWS_NONE;
 					t.p0[r] = in ? rw[r0 + r].second : 0;
 				}
-				recs[g].push_back(t);
+				so.recs[g].push_back(t);
 			}
 		const gemm_cblock &c0 = pl->cblocks[groups[g][0]];
 		double kt = 1;
 		for (uint32_t p = c0.pair_begin; p < c0.pair_end; p++)
 			kt += pl->pairs[p].k;
-		const uint32_t nt = (uint32_t)recs[g].size(), TR = tile_rect();
+		const uint32_t nt = (uint32_t)so.recs[g].size(), TR = tile_rect();
 		if (TR && nt >= 2 * TR && c0.q >= 2 * TR * BQ) {
 			for (uint32_t s0 = 0; s0 < nt; s0 += TR)
 				for (uint32_t q0 = 0; q0 < c0.q; q0 += TR * BQ) {
 					const uint32_t s1 = std::min(nt, s0 + TR);
 					const uint32_t q1 = std::min(c0.q,
 					    q0 + TR * BQ);
-					items.push_back({kt * (s1 - s0) *
+					so.items.push_back({kt * (s1 - s0) *
 					    ((q1 - q0 + BQ - 1) / BQ), g, q0, q1, s0,
 					    s1});
 				}
 			continue;
 		}
 		for (uint32_t q0 = 0; q0 < c0.q; q0 += BQ)
-			items.push_back({kt * (double)nt, g, q0,
+			so.items.push_back({kt * (double)nt, g, q0,
 			    std::min(c0.q, q0 + BQ), 0, nt});
 	}
-	std::vector<size_t> order(items.size());
+	std::vector<size_t> order(so.items.size());
 	for (size_t i = 0; i < order.size(); i++)
 		order[i] = i;
 	std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-		return items[a].work > items[b].work;
+		return so.items[a].work > so.items[b].work;
 	});
 	double load[NXCD] = {0};
-	std::vector<rec> bins[NXCD];
 	for (size_t o : order) {
 		const int x = (int)(std::min_element(load, load + NXCD) - load);
-		const item &it = items[o];
-		load[x] += it.work;
-		for (uint32_t q0 = it.q0; q0 < it.q1; q0 += BQ)
-			for (uint32_t s = it.s0; s < it.s1; s++) {
-				rec t = recs[it.g][s];
-				t.head.q0 = q0;
-				bins[x].push_back(t);
-			}
+		load[x] += so.items[o].work;
+		so.lists[x].push_back(o);
 	}
+}
+
+/* The stacked tiles in launch order (slot s of XCD x is workgroup
+ * 8 s + x); a tile of a split C block comes as adjacent halves, and `fix`
+ * gets the first halves (records of the same size). */
+template <int R>
+static void
+order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
+    std::vector<gemm_tile> &out, std::vector<gemm_tile> &fix)
+{
+	typedef typename stacked_order<R>::rec rec;
+	stacked_order<R> so;
+	stacked_items<R>(pl, BQ, merge, so);
+	std::vector<rec> bins[NXCD], fixes;
+	for (int x = 0; x < NXCD; x++)
+		for (size_t o : so.lists[x]) {
+			const typename stacked_order<R>::item &it = so.items[o];
+			for (uint32_t q0 = it.q0; q0 < it.q1; q0 += BQ)
+				for (uint32_t s = it.s0; s < it.s1; s++) {
+					rec t = so.recs[it.g][s];
+					t.head.q0 = q0;
+					if (pl->cblocks[t.head.cblock].split_pair ==
+					    KSPLIT_NONE) {
+						bins[x].push_back(t);
+						continue;
+					}
+					const uint32_t k = 1 + 2 *
+					    (uint32_t)fixes.size();
+					t.head.ksplit = k;
+					bins[x].push_back(t);
+					fixes.push_back(t);
+					t.head.ksplit = k + 1;
+					bins[x].push_back(t);
+				}
+		}
 	size_t per = 0;
 	for (auto &b : bins)
 		per = std::max(per, b.size());
@@ -1619,6 +1715,9 @@ order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
 	out.resize(flat.size() * (sizeof(rec) / sizeof(gemm_tile)));
 	if (!flat.empty())
 		memcpy(out.data(), flat.data(), flat.size() * sizeof(rec));
+	fix.resize(fixes.size() * (sizeof(rec) / sizeof(gemm_tile)));
+	if (!fixes.empty())
+		memcpy(fix.data(), fixes.data(), fixes.size() * sizeof(rec));
 }
 static_assert(sizeof(gemm_stile) == 3 * sizeof(gemm_tile),
     "a stacked FP64 tile is three tile records");
@@ -1633,14 +1732,22 @@ static_assert(sizeof(gemm_stile8) == 5 * sizeof(gemm_tile),
  * start and end stamps).  The C blocks holding the last KSPLIT_TAIL tiles
  * of each list are cut in two halves of their 16-k steps, which halves the
  * granularity where the lists drain; C blocks of fewer than
- * KSPLIT_MIN_STEPS steps are not cut.  The lists are those of 64 x 128
- * tiles whatever the candidates, so every tile shape cuts the same C
- * blocks at the same step and gives the same results.  Launches of more
- * than KSPLIT_MAX_TILES such tiles (over 12 per slot of the 768 that 3
- * workgroups on each of 256 CUs give) drain in a small share of their
- * time and are not split.
+ * KSPLIT_MIN_STEPS steps are not cut.  The lists are those of the stacked
+ * 128 x 128 tiles where a candidate is stacked (the last KSPLIT_TAIL_S
+ * tiles of each: the stacked tiles of the ladder, 4.4 a slot, left the
+ * slots idle 8.3% of the time, mostly in the last 3 ms; cutting the
+ * groups of the last 32, 64, 128 or 256 tiles of each list ran 16.22,
+ * 16.19, 16.23 and 16.49 ms against 16.29 uncut), else those of
+ * 64 x 128 tiles, whatever the candidates, so every tile shape cuts the
+ * same C blocks at the same step and gives the same results (a stacked
+ * tile cuts where its first C block does: the C blocks of a stack have the
+ * same pair sequence, and a group's C blocks are cut together).  Launches
+ * of more than KSPLIT_MAX_TILES 64 x 128 tiles (over 12 per slot of the
+ * 768 that 3 workgroups on each of 256 CUs give) drain in a small share of
+ * their time and are not split.
  */
 #define KSPLIT_TAIL 96
+#define KSPLIT_TAIL_S 64
 #define KSPLIT_MIN_STEPS 64
 #define KSPLIT_MAX_TILES (12 * 768)
 
@@ -1669,9 +1776,9 @@ mark_ksplit(contract_plan *pl)
 	}
 	if (!ksplit_enabled(pl))
 		return;
+	bool stacked = false;
 	for (int c = 0; c < pl->ncand; c++)
-		if (pl->cand_stacked[c])
-			return;		/* every candidate runs all of K */
+		stacked = stacked || pl->cand_stacked[c];
 	size_t ntiles = 0;
 	for (const gemm_cblock &cb : pl->cblocks)
 		ntiles += (size_t)((cb.p + BP - 1) / BP) * ((cb.q + BQ - 1) / BQ);
@@ -1694,6 +1801,24 @@ mark_ksplit(contract_plan *pl)
 			before += ns;
 		}
 	};
+	if (stacked) {
+		stacked_order<4> so;
+		stacked_items<4>(pl, 128, true, so);
+		const char *e = getenv("XM_KSPLIT_TAIL");
+		const uint32_t want = e && *e ? (uint32_t)atoi(e) :
+		    KSPLIT_TAIL_S;
+		for (auto &l : so.lists) {
+			uint32_t tail = 0;
+			for (size_t n = l.size(); n-- > 0 && tail < want;) {
+				const auto &it = so.items[l[n]];
+				tail += (it.s1 - it.s0) *
+				    ((it.q1 - it.q0 + 127) / 128);
+				for (uint32_t i : so.groups[it.g])
+					split(pl->cblocks[i]);
+			}
+		}
+		return;
+	}
 	if (grouped_order(pl)) {
 		std::vector<std::vector<uint32_t>> groups;
 		std::vector<tile_item> items;
@@ -1880,12 +2005,13 @@ build_tiles(contract_plan *pl)
 		if (pl->cand_stacked[c]) {
 			if (pl->precision == XM_PRECISION_SPLIT_INT8)
 				order_tiles_stacked<8>(pl, cand[c].second,
-				    stacked_crt(), pl->cand_tiles[c]);
+				    stacked_crt(), pl->cand_tiles[c],
+				    pl->cand_fix[c]);
 			else
 				order_tiles_stacked<4>(pl, cand[c].second,
 				    pl->precision == XM_PRECISION_NATIVE ||
-				    stacked_split(), pl->cand_tiles[c]);
-			pl->cand_fix[c].clear();
+				    stacked_split(), pl->cand_tiles[c],
+				    pl->cand_fix[c]);
 		} else if (grouped_order(pl))
 			order_tiles_grouped(pl, cand[c].first, cand[c].second,
 			    pl->cand_tiles[c], pl->cand_fix[c]);
@@ -1913,7 +2039,8 @@ select_tiles(contract_plan *pl)
 	pl->tiles = pl->cand_tiles[c];
 	pl->d_tiles = pl->d_cand_tiles[c];
 	pl->d_fix = pl->d_cand_fix[c];
-	pl->nfix = pl->cand_fix[c].size();
+	pl->nfix = pl->cand_fix[c].size() / (pl->cand_stacked[c] ?
+	    sizeof(gemm_stile) / sizeof(gemm_tile) : 1);
 	pl->tile_cfg = (int)(pl->cand_bp[c] * 1000 + pl->cand_bq[c]);
 }
 
@@ -1935,7 +2062,8 @@ gemm_tile_record(contract_plan *pl, float gemm_ms)
 		pl->tiles = pl->cand_tiles[c];
 		pl->d_tiles = pl->d_cand_tiles[c];
 		pl->d_fix = pl->d_cand_fix[c];
-		pl->nfix = pl->cand_fix[c].size();
+		pl->nfix = pl->cand_fix[c].size() / (pl->cand_stacked[c] ?
+		    sizeof(gemm_stile) / sizeof(gemm_tile) : 1);
 		pl->tile_cfg = (int)(pl->cand_bp[c] * 1000 + pl->cand_bq[c]);
 		return;
 	}
@@ -2014,10 +2142,15 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 		if (use_simple())
 			k_gemm_simple<double><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
-		else if (pl->stacked)
+		else if (pl->stacked) {
 			k_gemm_f64_dma_s<16, 2, 2><<<dim3((unsigned)(nt / 3)),
 			    dim3(FS_WG), 0, s>>>((const gemm_stile *)pl->d_tiles,
-			    pl->d_cblocks, pl->d_pairs, pl->d_scalars);
+			    pl->d_cblocks, pl->d_pairs, pl->d_scalars, pl->d_ksw);
+			if (pl->nfix)
+				k_ksplit_sum_s<<<dim3((unsigned)pl->nfix), b, 0,
+				    s>>>((const gemm_stile *)pl->d_fix,
+				    pl->d_cblocks, pl->d_ksw, pl->d_scalars);
+		}
 		else if (q128 && pl->f64_dma && persistent(pl)) {
 			static const unsigned nwg = resident_wgs(
 			    (const void *)k_gemm_f64_dma_q<16, 2, 3>);
@@ -2035,7 +2168,9 @@ launch_gemm(contract_plan *pl, hipStream_t s)
 		else
 			k_gemm_f64<64, 64, 16><<<g, b, 0, s>>>(pl->d_tiles,
 			    pl->d_cblocks, pl->d_pairs, pl->d_scalars, pl->d_ksw);
-		if (pl->nfix && q128)
+		if (pl->nfix && pl->stacked)
+			;	/* summed above */
+		else if (pl->nfix && q128)
 			k_ksplit_sum<64, 128><<<dim3((unsigned)pl->nfix), b, 0,
 			    s>>>(pl->d_fix, pl->d_cblocks, pl->d_ksw,
 			    pl->d_scalars);

@@ -10,6 +10,8 @@ C-panel case writes permuted C panels; alpha == 0 and beta == 0 keep
 libxm's rules exactly.  The BF16 split form's 128 x 128 tiles stack the
 same way (XM_SPLIT_STACK=0 keeps one C block a tile) and the int8 form's
 can (XM_CRT_STACK=1); each output's sums are unchanged, so the same bits.
+The stacked tiles of the groups that end the XCD lists run their K in two
+halves; the 64 x 128 form of the same plan cuts the same C blocks.
 """
 import os
 
@@ -196,3 +198,54 @@ def test_split_stacks_identical(gpulib, oracle, env, nslices):
         oracle.tensor_free(z)
     oracle.allocator_destroy(oal)
     assert rel_err(out["1"], want) <= 1e-12
+
+
+def test_stacked_ksplit(gpulib, oracle, env):
+    """K split of stacked tiles: the groups holding the last tiles of each
+    XCD list run as two halves of their 16-k steps, summed by
+    k_ksplit_sum_s.  The 64 x 128 candidate of the same plan cuts the same
+    C blocks at the same step: a plan's calls on either give the same bits
+    (beta = 0), within 1e-13 of the oracle; beta == 0 ignores NaN in C and
+    alpha == 0 writes beta * C exactly."""
+    env.pop("XM_GEMM_KSPLIT", None)
+    env.pop("XM_GEMM_TILE", None)
+    w = workloads.ladder_d2h(32, 192, 5)
+    alloc = gpulib.allocator()
+    t, v, c = (s.build(gpulib, alloc) for s in w.specs)
+    _fill(gpulib, w, (t, v, c), 31)
+    data = [s.read(gpulib, z) for s, z in zip(w.specs, (t, v, c))]
+    outs, kernels = [], []
+    for _ in range(4):
+        gpulib.contract(0.5, t, v, 0.0, c, *w.idx)
+        st = gpulib.stats()
+        assert st["ksplit_tiles"] > 0, st
+        kernels.append(st["gemm_kernel"])
+        outs.append(w.c.read(gpulib, c))
+    assert "k_gemm_f64_dma_s" in kernels and "k_gemm_f64_dma" in kernels, \
+        kernels
+    for o in outs[1:]:
+        for k in outs[0]:
+            assert np.array_equal(o[k], outs[0][k]), k
+    oal = oracle.allocator()
+    ot = [build_filled(oracle, oal, s, d) for s, d in zip(w.specs, data)]
+    oracle.contract(0.5, ot[0], ot[1], 0.0, ot[2], *w.idx)
+    want = w.c.read(oracle, ot[2])
+    assert rel_err(outs[0], want) <= 1e-13
+    # beta == 0 with NaN in C; alpha == 0 keeps beta * C
+    for k in w.c.canonical_blocks():
+        gpulib.write_block(c, k, np.full_like(np.asarray(want[k]), np.nan))
+    gpulib.contract(0.5, t, v, 0.0, c, *w.idx)
+    assert gpulib.stats()["gemm_kernel"] == kernels[-1]
+    got = w.c.read(gpulib, c)
+    for k in want:
+        assert np.array_equal(got[k], outs[0][k]), k
+    gpulib.contract(0.0, t, v, -2.0, c, *w.idx)
+    got2 = w.c.read(gpulib, c)
+    for k in want:
+        assert np.array_equal(got2[k], -2.0 * got[k]), k
+    for z in ot:
+        oracle.tensor_free(z)
+    oracle.allocator_destroy(oal)
+    for x in (t, v, c):
+        gpulib.tensor_free(x)
+    gpulib.allocator_destroy(alloc)
