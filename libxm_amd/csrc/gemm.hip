@@ -1801,12 +1801,13 @@ mark_ksplit(contract_plan *pl)
 			before += ns;
 		}
 	};
+	/* tiles at the end of each list whose C blocks are cut */
+	const char *e = getenv("XM_KSPLIT_TAIL");
+	const uint32_t want = e && *e ? (uint32_t)atoi(e) : stacked ?
+	    KSPLIT_TAIL_S : KSPLIT_TAIL;
 	if (stacked) {
 		stacked_order<4> so;
 		stacked_items<4>(pl, 128, true, so);
-		const char *e = getenv("XM_KSPLIT_TAIL");
-		const uint32_t want = e && *e ? (uint32_t)atoi(e) :
-		    KSPLIT_TAIL_S;
 		for (auto &l : so.lists) {
 			uint32_t tail = 0;
 			for (size_t n = l.size(); n-- > 0 && tail < want;) {
@@ -1826,7 +1827,7 @@ mark_ksplit(contract_plan *pl)
 		group_items(pl, BP, BQ, groups, items, lists);
 		for (auto &l : lists) {
 			uint32_t tail = 0;
-			for (size_t n = l.size(); n-- > 0 && tail < KSPLIT_TAIL;) {
+			for (size_t n = l.size(); n-- > 0 && tail < want;) {
 				const tile_item &it = items[l[n]];
 				for (uint32_t i : groups[it.g]) {
 					gemm_cblock &cb = pl->cblocks[i];
@@ -1842,7 +1843,7 @@ mark_ksplit(contract_plan *pl)
 	xcd_lists(pl, BP, BQ, lists);
 	for (auto &l : lists) {
 		uint32_t tail = 0;
-		for (size_t n = l.size(); n-- > 0 && tail < KSPLIT_TAIL;) {
+		for (size_t n = l.size(); n-- > 0 && tail < want;) {
 			gemm_cblock &cb = pl->cblocks[l[n]];
 			tail += ((cb.p + BP - 1) / BP) * ((cb.q + BQ - 1) / BQ);
 			split(cb);
