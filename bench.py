@@ -727,9 +727,12 @@ def main():
                      "frac": F / (gemm_ms * 1e-3) / 1e12 / FP64_PEAK,
                      # the ladder's panels are all P-contiguous with even
                      # extents: the DMA-staged kernels run it (the stacked
-                     # 128 x 128 form, or 64 x 128 tiles whose K-split
-                     # halves k_ksplit_sum adds)
+                     # 128 x 128 form or 64 x 128 tiles, whose K-split
+                     # halves k_ksplit_sum_s / k_ksplit_sum add)
                      "traffic": stage_traffic((st["gemm_kernel"],
+                                               "k_ksplit_sum_s"
+                                               if st["gemm_kernel"] ==
+                                               "k_gemm_f64_dma_s" else
                                                "k_ksplit_sum")
                                               if st["ksplit_tiles"] else
                                               (st["gemm_kernel"],)),
