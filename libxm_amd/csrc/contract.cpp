@@ -612,7 +612,9 @@ struct cinfo {
  * multiple of 4: every 4-k group the GEMM kernels hand to an MFMA then
  * lies within one term, as when each term is a pair of its own, so every
  * output sums the same products in the same groups and order, and the
- * results are the same bits (a zero's sign aside).  A class is formed
+ * results are the same bits (a zero's sign aside; the K split of the
+ * drain, where a plan has one, cuts a class's C block at its own middle
+ * step).  A class is formed
  * when the per-C-block form permutes all of its operand blocks and
  * computes its C blocks in panels too (so it adds no permute traffic) and
  * its padded MFMA work (rows in 32s, columns in 128s) is at least 8% below
