@@ -405,7 +405,8 @@ def ring_mode(lib, args):
 
 def precision_mode(lib, args):
     """Ladder o=64 v=512, entries U[-1,1]*10^U[-6,0]: FP64, split (6 and 7
-    slices) and FP32, each timed and compared with the FP64 GPU result."""
+    BF16 slices; the int8 CRT form with the moduli chosen per plan) and
+    FP32, each timed and compared with the FP64 GPU result."""
     from libxm_amd import xm, workloads
     from libxm_amd.spec import rel_err
     w = workloads.ladder_d2h(64, 512, args.seed, value_exp=(-6.0, 0.0))
@@ -419,19 +420,24 @@ def precision_mode(lib, args):
            "workload": "ladder o=64 v=512 D2h, U[-1,1]*10^U[-6,0]", "forms": {}}
     c1 = w.c.build(lib, alloc)
     results = {}
+    F = None
     for name, prec, ns in (("fp64", xm.PRECISION_NATIVE, None),
                            ("split6", xm.PRECISION_SPLIT_BF16, 6),
-                           ("split7", xm.PRECISION_SPLIT_BF16, 7)):
-        lib.set_precision(prec, ns)
+                           ("split7", xm.PRECISION_SPLIT_BF16, 7),
+                           ("int8", xm.PRECISION_SPLIT_INT8, None)):
+        lib.set_precision(prec, ns, 0 if name == "int8" else None)
         ms, gemm_ms, prep_ms, st = time_steps(lib, w, t, v, c, args.steps,
                                               max(args.warmup, 3))
         lib.contract(w.alpha, t, v, 0.0, c1, *w.idx)
         results[name] = w.c.read(lib, c1)
+        F = F or st["flops"]
         out["forms"][name] = {"ms_per_call": ms, "gemm_ms": gemm_ms,
-                              "slice_ms": prep_ms if ns else None,
-                              "tflops": st["flops"] / ms / 1e9}
+                              "slice_ms": prep_ms if prec else None,
+                              "tflops": F / ms / 1e9}
+        if name == "int8":
+            out["forms"][name]["moduli"] = st["nslices"]
+            out["forms"][name]["err_bound_model"] = st["err_bound"]
     lib.set_precision(xm.PRECISION_NATIVE)
-    F = st["flops"]
     wf = workloads.ladder_d2h(64, 512, args.seed, type_=xm.SCALAR_FLOAT)
     tf, vf, cf = (s.build(lib, alloc) for s in wf.specs)
     for dst, src in ((tf, t), (vf, v), (cf, c)):
@@ -441,7 +447,7 @@ def precision_mode(lib, args):
     results["fp32"] = wf.c.read(lib, cf)
     out["forms"]["fp32"] = {"ms_per_call": ms, "gemm_ms": gemm_ms,
                             "tflops": F / ms / 1e9}
-    for name in ("split6", "split7", "fp32"):
+    for name in ("split6", "split7", "int8", "fp32"):
         out["forms"][name]["rel_err_vs_fp64_gpu"] = rel_err(
             results[name], results["fp64"])
     out["flops"] = F
