@@ -10,7 +10,8 @@ term is a pair of its own: the class form gives the same bits as the
 per-C-block form (XM_CLASS_GEMM=0) and matches the oracle.  Also covered:
 alpha == 0 / beta == 0 rules, classes forced (XM_CLASS_GEMM=1) where the
 per-C-block form reads operands and C in place, the swapped orientation
-(C's b indices first), and a zero block, which splits a class.
+(C's b indices first), a zero block, which splits a class, and seeded
+random structures.
 """
 import os
 
@@ -194,3 +195,48 @@ def test_zero_block_splits_class(gpulib, oracle, env):
     _same_bits(got, ref)
     want, _ = _contract(oracle, specs, data, 1.0, 1.0, ("ik", "ka", "ia"))
     assert rel_err(got, want) <= 1e-13
+
+
+def _random_case(rng, t):
+    """A seeded contraction with uneven blocks, permuted storage and some
+    zero blocks (which break or split classes)."""
+    tmpl = [("ikac", "kbcj", "ijab"), ("ikac", "kbcj", "jbia"),
+            ("ick", "kca", "ia"), ("kic", "ack", "ai"), ("ik", "ka", "ia")]
+    ia, ib, ic = tmpl[rng.integers(len(tmpl))]
+    letters = sorted(set(ia + ib + ic))
+    ext = {x: int(rng.integers(12, 40)) for x in letters}
+    cuts = {}
+    for x in letters:
+        n = int(rng.integers(1, 4))
+        pts = sorted(set(int(v) for v in rng.integers(1, ext[x], n)))
+        cuts[x] = pts
+
+    def spec(idx, zero_frac):
+        s = Spec([ext[x] for x in idx], {k: cuts[x] for k, x in
+                                         enumerate(idx)}, t)
+        for blk in s.grid():
+            if rng.random() >= zero_frac:
+                s.canon(blk)
+        return s
+
+    return (spec(ia, 0.15), spec(ib, 0.15), spec(ic, 0.0)), (ia, ib, ic)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_structures(gpulib, oracle, env, seed):
+    """Seeded random block structures (uneven extents, zero blocks,
+    permuted storage of operands and C) in all four types: classes
+    forced and not give the same bits and match the oracle."""
+    rng = np.random.default_rng(1000 + seed)
+    t = list(NAMES)[seed % 4]
+    specs, idx = _random_case(rng, t)
+    data = _data(specs, seed)
+    al = round_scalar(complex(0.7, 0.2), t)
+    be = round_scalar(complex(-0.4, 0.1), t)
+    got, ref, st = _forms(gpulib, specs, data, al, be, idx, "1")
+    # forced, every C block with terms is in a class (one of its own at
+    # least: its terms then run as one K-concatenated pair)
+    assert st["classes"] > 0, st
+    _same_bits(got, ref)
+    want, _ = _contract(oracle, specs, data, al, be, idx)
+    assert rel_err(got, want) <= TOL[t], (idx, st["classes"])
