@@ -192,8 +192,8 @@ double crt_error_bound(const contract_plan *pl);
 int current_nmoduli(void);
 
 /* Whether a split contraction with this many slices runs the LDS-staged
- * kernel (128 x 128 tiles; slices 3-6) rather than the per-wave one (64 x 64
- * tiles; slices 7-8, or any count with XM_SPLIT_KERNEL=1). */
+ * kernel (128 x 128 tiles; slices 3-7) rather than the per-wave one (64 x 64
+ * tiles; 8 slices, or any count with XM_SPLIT_KERNEL=1). */
 bool split_staged_kernel(int nslices);
 
 #endif /* XM_CONTRACT_PLAN_H */

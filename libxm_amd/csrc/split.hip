@@ -217,7 +217,10 @@ ld_frag(const uint16_t *p)
  * fragment is a contiguous 1 KiB run, so it is one wave-wide
  * global_load_lds_dwordx4 (lane l's 16 bytes land at the LDS base + 16 l):
  * no staging registers.  NST stages rotate with the DMA running NST - 1
- * steps ahead; every wave issues exactly S + 1 DMAs per step (row tiles
+ * steps ahead (three stages up to S = 6; two at S = 7, whose 56 KiB stages
+ * do not fit three times: on the o=64 v=384 ladder its GEMM then ran
+ * 21.8 ms against 35.5 on the per-wave kernel, at 255 VGPRs and no
+ * spills); every wave issues exactly S + 1 DMAs per step (row tiles
  * past an operand's padding read a zero page), so the end of a step waits
  * with a constant count for the next step's data and leaves the later
  * steps' loads in flight; one barrier per step.  The (pair, k step)
@@ -799,7 +802,7 @@ split_staged_kernel(int nslices)
 		const char *e = getenv("XM_SPLIT_KERNEL");
 		reference = e && *e == '1';
 	}
-	return !reference && nslices >= 3 && nslices <= 6;
+	return !reference && nslices >= 3 && nslices <= 7;
 }
 
 void
@@ -846,7 +849,8 @@ split_gemm(contract_plan *pl, hipStream_t s)
 		case 3: k_gemm_split2<3><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
 		case 4: k_gemm_split2<4><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
 		case 5: k_gemm_split2<5><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
-		default: k_gemm_split2<6><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
+		case 6: k_gemm_split2<6><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
+		default: k_gemm_split2<7><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
 		}
 	}
 	hipError_t e = hipGetLastError();
