@@ -79,6 +79,7 @@ struct contract_plan {
 	bool mfma_dw = false;		/* k_gemm_mfma_dw can run the plan */
 	bool mfma_x4 = false;		/* ... with 16-byte DMAs */
 	bool even_extents = false;	/* every C block's P and Q even */
+	bool split_staged = false;	/* SPLIT_BF16 on k_gemm_split2 */
 	/* plans of xm_contract_noalloc: one tile shape (the first
 	 * candidate, never tuned) and no K split, whose partial sums would
 	 * need device memory of their own */
@@ -192,8 +193,8 @@ double crt_error_bound(const contract_plan *pl);
 int current_nmoduli(void);
 
 /* Whether a split contraction with this many slices runs the LDS-staged
- * kernel (128 x 128 tiles; slices 3-7) rather than the per-wave one (64 x 64
- * tiles; 8 slices, or any count with XM_SPLIT_KERNEL=1). */
+ * kernel (128 x 128 tiles; slices 3-8) rather than the per-wave one (64 x 64
+ * tiles; with XM_SPLIT_KERNEL=1). */
 bool split_staged_kernel(int nslices);
 
 #endif /* XM_CONTRACT_PLAN_H */

@@ -1284,7 +1284,7 @@ tile_candidates(const contract_plan *pl, std::vector<std::pair<uint32_t,
     uint32_t>> &out)
 {
 	if (pl->precision == XM_PRECISION_SPLIT_BF16) {
-		out.push_back(split_staged_kernel(pl->nslices) ?
+		out.push_back(pl->split_staged ?
 		    std::make_pair(128u, 128u) : std::make_pair(64u, 64u));
 	} else if (pl->precision == XM_PRECISION_SPLIT_INT8) {
 		out.push_back({256, 256});
@@ -1984,6 +1984,10 @@ stacked_split(void)
 void
 build_tiles(contract_plan *pl)
 {
+	/* the split kernel is the plan's for good: its tile records depend
+	 * on it */
+	pl->split_staged = pl->precision == XM_PRECISION_SPLIT_BF16 &&
+	    split_staged_kernel(pl->nslices);
 	pl->f64_dma = f64_dma_eligible(pl);
 	pl->mfma_dw = mfma_dw_eligible(pl);
 	pl->mfma_x4 = mfma_x4_eligible(pl);

@@ -217,10 +217,11 @@ ld_frag(const uint16_t *p)
  * fragment is a contiguous 1 KiB run, so it is one wave-wide
  * global_load_lds_dwordx4 (lane l's 16 bytes land at the LDS base + 16 l):
  * no staging registers.  NST stages rotate with the DMA running NST - 1
- * steps ahead (three stages up to S = 6; two at S = 7, whose 56 KiB stages
- * do not fit three times: on the o=64 v=384 ladder its GEMM then ran
- * 21.8 ms against 35.5 on the per-wave kernel, at 255 VGPRs and no
- * spills); every wave issues exactly S + 1 DMAs per step (row tiles
+ * steps ahead (three stages up to S = 6; two at S = 7 and 8, whose 56 and
+ * 64 KiB stages do not fit three times: on the o=64 v=384 ladder their
+ * GEMMs then ran 21.8 and 26.5 ms against 35.5 and 44.3 on the per-wave
+ * kernel, at 255 and 256 VGPRs, S = 8 spilling one); every wave issues
+ * exactly S + 1 DMAs per step (row tiles
  * past an operand's padding read a zero page), so the end of a step waits
  * with a constant count for the next step's data and leaves the later
  * steps' loads in flight; one barrier per step.  The (pair, k step)
@@ -797,12 +798,8 @@ split_prepare(contract_plan *pl, const xm_tensor_t *, const xm_tensor_t *,
 bool
 split_staged_kernel(int nslices)
 {
-	static int reference = -1;
-	if (reference < 0) {
-		const char *e = getenv("XM_SPLIT_KERNEL");
-		reference = e && *e == '1';
-	}
-	return !reference && nslices >= 3 && nslices <= 7;
+	const char *e = getenv("XM_SPLIT_KERNEL");
+	return !(e && *e == '1') && nslices >= 3 && nslices <= 8;
 }
 
 void
@@ -838,7 +835,7 @@ split_gemm(contract_plan *pl, hipStream_t s)
 	const gemm_tile *t = pl->d_tiles;
 	const gemm_cblock *c = pl->d_cblocks;
 	const gemm_scalars *sc = pl->d_scalars;
-	if (!split_staged_kernel(ss->nslices)) {
+	if (!pl->split_staged) {
 		SDISPATCH(k_gemm_split, ss->nslices, g, dim3(SPLIT_WG), 0, s)(t,
 		    c, ss->d_pairs, sc);
 	} else {
@@ -850,7 +847,8 @@ split_gemm(contract_plan *pl, hipStream_t s)
 		case 4: k_gemm_split2<4><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
 		case 5: k_gemm_split2<5><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
 		case 6: k_gemm_split2<6><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
-		default: k_gemm_split2<7><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
+		case 7: k_gemm_split2<7><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
+		default: k_gemm_split2<8><<<g3, b, 0, s>>>(ts, c, ss->d_pairs, sc); break;
 		}
 	}
 	hipError_t e = hipGetLastError();
@@ -858,7 +856,7 @@ split_gemm(contract_plan *pl, hipStream_t s)
 		xm_fatal("xm_contract: split GEMM launch failed: %s",
 		    hipGetErrorString(e));
 	pl->gemm_launches = 1;
-	pl->gemm_kernel = split_staged_kernel(ss->nslices) ? "k_gemm_split2" :
+	pl->gemm_kernel = pl->split_staged ? "k_gemm_split2" :
 	    "k_gemm_split";
 }
 

@@ -102,3 +102,20 @@ def test_split_alpha_zero_and_table_case(gpulib, oracle, split):
             want = run_contract(oracle, specs, data, al, be, idx)
             assert rel_err(got, want) <= 1e-12, (fn.__name__, al, be)
 
+
+
+@pytest.mark.parametrize("n", [7, 8])
+@pytest.mark.parametrize("kernel", ["k_gemm_split2", "k_gemm_split"])
+def test_split_many_slices(gpulib, oracle, split, monkeypatch, n, kernel):
+    """7 and 8 slices on the LDS-staged kernel (two stages) and on the
+    per-wave reference kernel (XM_SPLIT_KERNEL=1, fixed per plan), on
+    six-decade data: both within 1e-12 of the oracle."""
+    monkeypatch.setenv("XM_SPLIT_KERNEL", "1" if kernel == "k_gemm_split"
+                       else "0")
+    w = workloads.ladder_d2h(32, 128, 2)
+    data = data_for(w.specs, 14 + n, wide=True, pairs=w.pairs)
+    want = run_contract(oracle, w.specs, data, w.alpha, 0.0, w.idx)
+    split(n)
+    got, st = run_contract(gpulib, w.specs, data, w.alpha, 0.0, w.idx, True)
+    assert st["gemm_kernel"] == kernel and st["nslices"] == n, st
+    assert rel_err(got, want) <= 1e-12
