@@ -288,7 +288,8 @@ k_crt_quant(const crt_inst *__restrict__ insts, uint32_t ninst,
 /* ---- GEMM ------------------------------------------------------------ */
 
 /*
- * Workgroup tile 256 (P) x 256 (Q), 8 waves in 4 x 2 (two a SIMD), each
+ * Workgroup tile 256 (P) x 256 (Q), one part of the moduli (head.ksplit:
+ * parts << 8 | part + 1, 0 for all), 8 waves in 4 x 2 (two a SIMD), each
  * 64 x 128 as 2 x 4 MFMA tiles of 32 x 32 (128 int32 accumulators a
  * lane); one workgroup per CU.  A tile record (gemm_stile8) names the
  * C block and first row of each of its eight 32-row tiles: the rows of one
@@ -363,6 +364,12 @@ k_gemm_crt(const gemm_stile8 *__restrict__ tiles,
 	const gemm_tile tl = tp->head;
 	if (tl.cblock == 0xffffffffu)
 		return;
+	const uint32_t ns = tl.ksplit ? tl.ksplit >> 8 : 1;
+	const uint32_t pj = tl.ksplit ? (tl.ksplit & 0xff) - 1 : 0;
+	const uint32_t m0 = pj * (uint32_t)nmod / ns;
+	const uint32_t m1 = (pj + 1) * (uint32_t)nmod / ns;
+	if (m0 >= m1)
+		return;		/* a part with no modulus */
 	const gemm_scalars sc = *scp;
 	/* the first C block of the tile's stack: its pairs give Y, k steps
 	 * and factor ratios, shared by every C block of the stack */
@@ -427,9 +434,9 @@ k_gemm_crt(const gemm_stile8 *__restrict__ tiles,
 			    (cr_lds_t *)&nx->frag[((8 + wave) * CR_KQ + h) * 64],
 			    16, 0, 0);
 	};
-	uint32_t fl = 0, fi = pbeg, fk = 0;
+	uint32_t fl = m0, fi = pbeg, fk = 0;
 	crt_pair fp = pairs[pbeg];
-	setup(fp, 0, pbeg);
+	setup(fp, m0, pbeg);
 	bool fmore = true;
 	auto fadvance = [&]() {
 		if (++fk < fp.nsteps)
@@ -437,7 +444,7 @@ k_gemm_crt(const gemm_stile8 *__restrict__ tiles,
 		fk = 0;
 		if (++fi == pend) {
 			fi = pbeg;
-			if (++fl == (uint32_t)nmod) {
+			if (++fl == m1) {
 				fmore = false;
 				return;
 			}
@@ -553,7 +560,7 @@ k_gemm_crt(const gemm_stile8 *__restrict__ tiles,
 		__atomic_signal_fence(__ATOMIC_SEQ_CST);
 	};
 	int cs = 0;
-	for (uint32_t l = 0; l < (uint32_t)nmod; l++) {
+	for (uint32_t l = m0; l < m1; l++) {
 		for (uint32_t i = pbeg; i < pend; i++) {
 			const crt_pair cp = pairs[i];
 			if (i != pbeg && cp.ratio != 1)

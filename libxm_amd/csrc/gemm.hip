@@ -1566,6 +1566,8 @@ order_tiles_grouped(const contract_plan *pl, uint32_t BP, uint32_t BQ,
  * dealt to the XCD lists longest first, as in the grouped order, so the
  * workgroups an XCD runs together read the same Y fragments.
  */
+static uint32_t crt_moduli_parts(void);
+
 template <int R>
 struct stacked_order {
 	struct rec {
@@ -1680,6 +1682,7 @@ order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
 	typedef typename stacked_order<R>::rec rec;
 	stacked_order<R> so;
 	stacked_items<R>(pl, BQ, merge, so);
+	const uint32_t nsp = R == 8 ? crt_moduli_parts() : 1;
 	std::vector<rec> bins[NXCD], fixes;
 	for (int x = 0; x < NXCD; x++)
 		for (size_t o : so.lists[x]) {
@@ -1688,6 +1691,15 @@ order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
 				for (uint32_t s = it.s0; s < it.s1; s++) {
 					rec t = so.recs[it.g][s];
 					t.head.q0 = q0;
+					if (R == 8 && nsp > 1) {
+						/* int8: one tile a part of
+						 * the moduli, run together */
+						for (uint32_t j = 0; j < nsp; j++) {
+							t.head.ksplit = nsp << 8 | (j + 1);
+							bins[x].push_back(t);
+						}
+						continue;
+					}
 					if (pl->cblocks[t.head.cblock].split_pair ==
 					    KSPLIT_NONE) {
 						bins[x].push_back(t);
@@ -1970,6 +1982,25 @@ stacked_crt(void)
 {
 	const char *e = getenv("XM_CRT_STACK");
 	return e && *e == '1';
+}
+
+/*
+ * The int8 form's tiles each run a part of the moduli (their residue
+ * planes are independent; the reconstruction reads them all): tile t of
+ * a launch computes moduli [j n / parts, (j + 1) n / parts) for part j in
+ * head.ksplit (parts << 8 | j + 1; 0: all), the parts of a tile adjacent
+ * in the list.  Twice the tiles of half the length let the XCD lists
+ * drain more evenly at no extra traffic (each plane is still staged once
+ * a tile): on the o=64 v=384 ladder 2 parts ran the GEMM 2% faster than
+ * 1, 3 parts as fast as 1, 4 and 6 slower.  XM_CRT_MSPLIT sets the parts
+ * (default 2, 1 to 8).
+ */
+static uint32_t
+crt_moduli_parts(void)
+{
+	const char *e = getenv("XM_CRT_MSPLIT");
+	const int v = e && *e ? atoi(e) : 2;
+	return (uint32_t)std::min(8, std::max(1, v));
 }
 
 /* The BF16 split form's 128 x 128 tiles stack a group's C blocks like the

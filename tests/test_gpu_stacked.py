@@ -30,7 +30,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture
 def env():
     keys = ("XM_GEMM_KSPLIT", "XM_GEMM_TILE", "XM_CRT_STACK",
-            "XM_SPLIT_STACK")
+            "XM_SPLIT_STACK", "XM_CRT_MSPLIT")
     old = {k: os.environ.get(k) for k in keys}
     yield os.environ
     for k, v in old.items():
@@ -140,12 +140,15 @@ def test_stacked_c_panels(gpulib, oracle, env):
 
 def test_int8_stacks_identical(gpulib, env):
     """The int8 form with a group's C blocks stacked in shared tiles
-    (XM_CRT_STACK=1) against one C block a tile: the residue products are
-    exact integers, so the results are the same bits."""
+    (XM_CRT_STACK=1) against one C block a tile, and with each tile's
+    moduli in 1, 2 (default) or 3 parts (XM_CRT_MSPLIT): the residue
+    products are exact integers, so the results are the same bits."""
     w = workloads.ladder_d2h(64, 128, 2)
     out = {}
-    for flag in ("0", "1"):
+    for flag, parts in (("0", "1"), ("1", "2"), ("0", "2"), ("0", "3")):
         env["XM_CRT_STACK"] = flag
+        env["XM_CRT_MSPLIT"] = parts
+        flag = flag + parts
         alloc = gpulib.allocator()
         x = [s.build(gpulib, alloc) for s in w.specs]
         _fill(gpulib, w, x, 17)
@@ -157,8 +160,9 @@ def test_int8_stacks_identical(gpulib, env):
         for z in x:
             gpulib.tensor_free(z)
         gpulib.allocator_destroy(alloc)
-    for k in out["0"]:
-        assert np.array_equal(out["0"][k], out["1"][k]), k
+    for f in ("12", "02", "03"):
+        for k in out["01"]:
+            assert np.array_equal(out["01"][k], out[f][k]), (f, k)
 
 
 @pytest.mark.parametrize("nslices", [5, 6])
