@@ -1687,19 +1687,24 @@ order_tiles_stacked(const contract_plan *pl, uint32_t BQ, bool merge,
 	for (int x = 0; x < NXCD; x++)
 		for (size_t o : so.lists[x]) {
 			const typename stacked_order<R>::item &it = so.items[o];
+			if (R == 8 && nsp > 1) {
+				/* int8: the item's tiles part by part, so the
+				 * tiles running together read the same
+				 * moduli's planes */
+				for (uint32_t j = 0; j < nsp; j++)
+					for (uint32_t q0 = it.q0; q0 < it.q1; q0 += BQ)
+						for (uint32_t s = it.s0; s < it.s1; s++) {
+							rec t = so.recs[it.g][s];
+							t.head.q0 = q0;
+							t.head.ksplit = nsp << 8 | (j + 1);
+							bins[x].push_back(t);
+						}
+				continue;
+			}
 			for (uint32_t q0 = it.q0; q0 < it.q1; q0 += BQ)
 				for (uint32_t s = it.s0; s < it.s1; s++) {
 					rec t = so.recs[it.g][s];
 					t.head.q0 = q0;
-					if (R == 8 && nsp > 1) {
-						/* int8: one tile a part of
-						 * the moduli, run together */
-						for (uint32_t j = 0; j < nsp; j++) {
-							t.head.ksplit = nsp << 8 | (j + 1);
-							bins[x].push_back(t);
-						}
-						continue;
-					}
 					if (pl->cblocks[t.head.cblock].split_pair ==
 					    KSPLIT_NONE) {
 						bins[x].push_back(t);
@@ -1988,12 +1993,13 @@ stacked_crt(void)
  * The int8 form's tiles each run a part of the moduli (their residue
  * planes are independent; the reconstruction reads them all): tile t of
  * a launch computes moduli [j n / parts, (j + 1) n / parts) for part j in
- * head.ksplit (parts << 8 | j + 1; 0: all), the parts of a tile adjacent
- * in the list.  Twice the tiles of half the length let the XCD lists
- * drain more evenly at no extra traffic (each plane is still staged once
- * a tile): on the o=64 v=384 ladder 2 parts ran the GEMM 2% faster than
- * 1, 3 parts as fast as 1, 4 and 6 slower.  XM_CRT_MSPLIT sets the parts
- * (default 2, 1 to 8).
+ * head.ksplit (parts << 8 | j + 1; 0: all), an item's tiles listed part
+ * by part, so the tiles running together read the same moduli's planes.
+ * Twice the tiles of half the length let the XCD lists drain more evenly
+ * (each plane is still staged once a tile): on the o=64 v=384 ladder 2
+ * parts ran the GEMM 2.2% faster than 1 (the parts of a tile adjacent
+ * instead: 1.6%, with twice the HBM reads), 3 parts as fast as 1, 4 and 6
+ * slower.  XM_CRT_MSPLIT sets the parts (default 2, 1 to 8).
  */
 static uint32_t
 crt_moduli_parts(void)
